@@ -1,0 +1,182 @@
+"""RK3 steps/s of the channel DNS time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+                    [--config c3] [--no-cpu]
+
+Workload: the production 384-class grid (nx=191, ny=384, nz=189, a=1.5,
+alfa0=0.5, beta0=1, Re=12431, CPI type 1 with gamma=0.161436, cflmax=1),
+seeded perturbed-laminar start, state resident in HBM.  Times exactly K RK3
+steps after W warm-up steps, bracketed by device synchronisation, and prints
+one JSON line.  Per-kernel durations come from HIP events that the library
+records on its own stream around every launch during the timed steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def workload(name):
+    from channel_amd.dns import DnsConfig
+    if name == "c3":
+        return "channel 191x384x189 (nx, ny, nz), Re 12431, CPI gamma 0.161436, cflmax 1", DnsConfig(
+            nx=191, ny=384, nz=189, alfa0=0.5, beta0=1.0, ni=1.0 / 12431.0, a=1.5, cpi=True, cpi_type=1,
+            gamma=0.161436, deltat=0.0, cflmax=1.0)
+    if name == "c2":
+        return "channel 95x128x95 (nx, ny, nz), Re 3000, constant flow rate, dt 0.004", DnsConfig(
+            nx=95, ny=128, nz=95, alfa0=1.0, beta0=2.0, ni=1.0 / 3000.0, a=1.5, meanflowx=4.0 / 3.0, deltat=0.004)
+    if name == "c1":
+        return "channel 31x64x31 (nx, ny, nz), Re 3000, constant flow rate, dt 0.004", DnsConfig(
+            nx=31, ny=64, nz=31, alfa0=1.0, beta0=2.0, ni=1.0 / 3000.0, a=1.5, meanflowx=4.0 / 3.0, deltat=0.004)
+    if name == "c4":
+        return "channel 511x512x511 (nx, ny, nz), Re 20000, constant flow rate, dt 0.002", DnsConfig(
+            nx=511, ny=512, nz=511, alfa0=0.5, beta0=1.0, ni=1.0 / 20000.0, a=1.5, meanflowx=4.0 / 3.0, deltat=0.002)
+    raise SystemExit("unknown workload %s" % name)
+
+
+def algorithmic_bytes(cfg, nxd, nzd, c=16):
+    """Algorithmic HBM bytes of one substep per kernel (each launch covers the
+    whole volume).  S64: one FP64 spectral plane of one component; Sc, Pc:
+    spectral / z-physical planes of the transform intermediates."""
+    Nx, Nz, Ny, Nyi = cfg.nx + 1, 2 * cfg.nz + 1, cfg.ny + 3, cfg.ny - 1
+    S64 = Nx * Nz * 16
+    Sc = Nx * Nz * c
+    Pc = Nx * nzd * c
+    per = {
+        "zinv": Ny * (3 * S64 + 3 * Pc),
+        "xpass": Ny * (3 * Pc + 6 * Pc),
+        "zfwd": Ny * (6 * Pc + 6 * Sc),
+        "rhs": Ny * (6 * Sc + 3 * S64) + Nyi * (4 * S64 + 2 * S64),
+        "solve": Nyi * 2 * S64 + Ny * 3 * S64,
+    }
+    step = 3 * (Ny * (9 * S64 + 12 * Sc + 18 * Pc) + 8 * Nyi * S64)
+    assert abs(3 * sum(per.values()) - step) < 1e-6 * step
+    return per, step
+
+
+def cpu_baseline(cfg, V, nplanes=32, nix=12):
+    """The oracle (plain C restatement of the reference step, OpenMP over
+    planes and x modes) timed on the host cores on a bounded sample of the
+    same workload, extrapolated to RK3 steps/s."""
+    from oracle import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    orc.set_threads(threads)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    t0 = time.time()
+    tc, tr, ts = o.time_sample(nplanes, nix)
+    wall = time.time() - t0
+    o.close()
+    Ny, Nyi, Nx = cfg.ny + 3, cfg.ny - 1, cfg.nx + 1
+    t_step = 3 * (Ny * tc + Nyi * tr + Nx * ts)
+    return {
+        "value": 1.0 / t_step,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "products of %d planes, rhs of %d rows, solve of %d x-mode columns of one substep (%.1f s wall), "
+                  "scaled to %d planes, %d rows, %d columns x 3 substeps" % (nplanes, nplanes - 4, nix, wall, Ny, Nyi, Nx),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip per-kernel events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from channel_amd import lib
+    from channel_amd.initial import perturbed_laminar
+    from channel_amd.solver import Channel
+
+    wname, cfg = workload(args.config)
+    prec = lib.CHAN_FP64 if args.precision == "fp64" else lib.CHAN_SPLIT
+    V = perturbed_laminar(cfg)
+    ch = Channel(cfg, prec)
+    ch.set_field(V)
+    ch.initial()
+    ch.outstats()
+    for _ in range(args.warmup):
+        ch.step()
+    if not args.no_timing:
+        ch.timing(True)
+    ch.sync()
+    t0 = time.perf_counter()
+    rows = ch.steps(args.steps)
+    ch.sync()
+    t1 = time.perf_counter()
+    ktimes = ch.kernel_times() if not args.no_timing else {}
+    ch.timing(False)
+    elapsed = t1 - t0
+    if not np.all(np.isfinite(rows)):
+        raise SystemExit("non-finite Runtimedata in the timed steps")
+    sps = args.steps / elapsed
+
+    per, step_bytes = algorithmic_bytes(cfg, ch.nxd, ch.nzd, 16 if prec == lib.CHAN_FP64 else 8)
+    # kernel-level roofline of the dominant kernel
+    roof = None
+    if ktimes:
+        groups = {"zinv": ["zinv"], "xpass": ["xpass"], "zfwd": ["zfwd"], "rhs": ["rhs"],
+                  "solve": ["solve_ve", "mean", "uw"]}
+        best = None
+        for g, ks in groups.items():
+            tot = sum(ktimes.get(k, (0, 0.0))[1] for k in ks)
+            cnt = max(ktimes.get(k, (0, 0.0))[0] for k in ks)
+            if cnt and (best is None or tot > best[1]):
+                best = (g, tot, cnt)
+        g, tot, cnt = best
+        avg_s = tot / cnt / 1e3
+        ach = per[g] / avg_s
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (g, args.config))
+        if os.path.exists(tfile):
+            traffic = json.load(open(tfile)).get("hbm_bytes_per_launch")
+        roof = {"kernel": g, "bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": traffic, "algorithmic_bytes": per[g], "avg_ms": avg_s * 1e3,
+                "launches": cnt}
+
+    out = {
+        "metric": "RK3 steps/s at 191x384x189, FP64 vs FP32/FP64 split; % HBM roofline",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == lib.CHAN_FP64 else "f32+f64",
+        "data": "synthetic: seeded perturbed-laminar field (splitmix64 seed 20261202)",
+        "config": {"workload": wname, "precision": args.precision, "nxd": ch.nxd, "nzd": ch.nzd},
+        "step_roofline": {"bytes_per_step": step_bytes, "achieved_GBps": step_bytes * sps / 1e9,
+                          "frac": step_bytes * sps / HBM_PEAK},
+        "roofline": roof,
+        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in sorted(ktimes.items())},
+    }
+    ch.close()
+    if not args.no_cpu and args.gpus == 1:
+        out["cpu_baseline"] = cpu_baseline(cfg, V)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,669 @@
+// api.hip -- the C-ABI of include/channel_hip.h: device-resident state of
+// one channel DNS, set-up, field exchange in the reference layout and the
+// launch sequences of buildrhs / linsolve / the RK3 time loop.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/channel_hip.h"
+#include "chan_internal.hpp"
+
+using namespace chan;
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) return fail((int)e_, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+// RK3 coefficients of Rai & Moin (dnsdata.f90:70-72)
+const double RK1[3] = {120.0 / 32.0, 2.0, 0.0};
+const double RK2[3] = {120.0 / 8.0, 50.0 / 8.0, 34.0 / 8.0};
+const double RK3[3] = {120.0 / 20.0, 90.0 / 20.0, 50.0 / 20.0};
+
+// ---------------------------------------------------- layout transposes --
+// host layout: [c][pair][row] (the reference's V(iy,iz,ix,c), iy fastest);
+// device layout: [c][row][pair] with plane stride ps.
+constexpr int TT = 32;
+__global__ __launch_bounds__(TT * 8) void k_to_dev(const cx<double> *__restrict__ in, cx<double> *__restrict__ out, Geom g) {
+  __shared__ cx<double> tile[TT][TT + 1];
+  const int c = blockIdx.z;
+  const int p0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
+  const cx<double> *src = in + (size_t)c * g.npair * g.Ny;
+  cx<double> *dst = out + (size_t)c * g.Ny * g.ps;
+  for (int k = threadIdx.y; k < TT; k += 8) {
+    const int p = p0 + k, r = r0 + threadIdx.x;
+    if (p < g.npair && r < g.Ny) tile[k][threadIdx.x] = src[(size_t)p * g.Ny + r];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < TT; k += 8) {
+    const int r = r0 + k, p = p0 + threadIdx.x;
+    if (p < g.npair && r < g.Ny) dst[(size_t)r * g.ps + p] = tile[threadIdx.x][k];
+  }
+}
+__global__ __launch_bounds__(TT * 8) void k_to_host(const cx<double> *__restrict__ in, cx<double> *__restrict__ out, Geom g) {
+  __shared__ cx<double> tile[TT][TT + 1];
+  const int c = blockIdx.z;
+  const int p0 = blockIdx.x * TT, r0 = blockIdx.y * TT;
+  const cx<double> *src = in + (size_t)c * g.Ny * g.ps;
+  cx<double> *dst = out + (size_t)c * g.npair * g.Ny;
+  for (int k = threadIdx.y; k < TT; k += 8) {
+    const int r = r0 + k, p = p0 + threadIdx.x;
+    if (p < g.npair && r < g.Ny) tile[k][threadIdx.x] = src[(size_t)r * g.ps + p];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < TT; k += 8) {
+    const int p = p0 + k, r = r0 + threadIdx.x;
+    if (p < g.npair && r < g.Ny) dst[(size_t)p * g.Ny + r] = tile[threadIdx.x][k];
+  }
+}
+
+// (0,0) columns of all three components: out[c][row]
+__global__ void k_gather00(const cx<double> *__restrict__ V, cx<double> *__restrict__ out, Geom g) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.Ny) return;
+  for (int c = 0; c < 3; c++) out[c * g.Ny + r] = V[((size_t)c * g.Ny + r) * g.ps + g.nz];
+}
+
+void twiddles(std::vector<cx<double>> &w, int n) {
+  w.resize(n);
+  for (int e = 0; e < n; e++) {
+    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
+    w[e] = cx<double>{(double)cosl(ang), (double)sinl(ang)};
+  }
+}
+
+}  // namespace
+
+namespace chan {
+hipError_t launch_to_device_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g) {
+  dim3 grid((unsigned)((g.npair + TT - 1) / TT), (unsigned)((g.Ny + TT - 1) / TT), 3);
+  hipLaunchKernelGGL(k_to_dev, grid, dim3(TT, 8), 0, st, in, out, g);
+  return hipGetLastError();
+}
+hipError_t launch_to_host_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g) {
+  dim3 grid((unsigned)((g.npair + TT - 1) / TT), (unsigned)((g.Ny + TT - 1) / TT), 3);
+  hipLaunchKernelGGL(k_to_host, grid, dim3(TT, 8), 0, st, in, out, g);
+  return hipGetLastError();
+}
+}  // namespace chan
+
+struct chan_ctx {
+  chan_params prm;
+  HostSetup hs;
+  Geom g;
+  hipStream_t st = nullptr;
+  // device state
+  cx<double> *V = nullptr, *old = nullptr, *rhs = nullptr, *ucor = nullptr, *col00 = nullptr;
+  double2 *L = nullptr;
+  cx<double> *A = nullptr, *B = nullptr, *P = nullptr;  // FP64 transform intermediates
+  cx<double> *twz = nullptr, *twx = nullptr;
+  double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr, *cflpart = nullptr;
+  DevScalars *sc = nullptr;
+  size_t ncflpart = 0;
+  // host scalars
+  double time = 0, deltat = 0;
+  bool field_set = false;
+  // timing
+  bool timing = false;
+  struct Rec {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pend;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::string> names;
+  std::vector<long long> counts;
+  std::vector<double> ms;
+};
+
+namespace {
+
+int kid(chan_ctx *c, const char *name) {
+  for (size_t i = 0; i < c->names.size(); i++)
+    if (c->names[i] == name) return (int)i;
+  c->names.push_back(name);
+  c->counts.push_back(0);
+  c->ms.push_back(0.0);
+  return (int)c->names.size() - 1;
+}
+
+hipEvent_t get_event(chan_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int harvest(chan_ctx *c) {
+  for (auto &r : c->pend) {
+    HIPCHK(hipEventSynchronize(r.b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+    c->counts[r.id] += 1;
+    c->ms[r.id] += t;
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->pend.clear();
+  return 0;
+}
+
+// runs `launch` on the context's stream, bracketed by events when timing
+template <class F>
+int timed(chan_ctx *c, const char *name, F launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    if (c->pend.size() > 8192) {
+      int rc = harvest(c);
+      if (rc) return rc;
+    }
+    a = get_event(c);
+    b = get_event(c);
+    if (!a || !b) return fail(CHAN_E_STATE, "hipEventCreate failed");
+    HIPCHK(hipEventRecord(a, c->st));
+  }
+  hipError_t e = launch();
+  if (e != hipSuccess) return fail((int)e, "launch of %s failed: %s", name, hipGetErrorString(e));
+  if (c->timing) {
+    HIPCHK(hipEventRecord(b, c->st));
+    c->pend.push_back({kid(c, name), a, b});
+  }
+  return 0;
+}
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 256);
+  if (e != hipSuccess) return fail((int)e, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  return 0;
+}
+
+double cfl_value(const DevScalars &s) {
+  double v;
+  std::memcpy(&v, &s.cfl_bits, sizeof v);
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int chan_abi_version(void) { return CHAN_ABI_VERSION; }
+const char *chan_last_error(void) { return g_err; }
+
+int chan_create(const chan_params *p, chan_ctx **out) {
+  if (!p || !out) return fail(CHAN_E_ARG, "null argument");
+  *out = nullptr;
+  if (p->nx < 1 || p->nz < 1 || p->ny < 8) return fail(CHAN_E_ARG, "need nx >= 1, nz >= 1, ny >= 8 (got %d %d %d)", p->nx, p->ny, p->nz);
+  if (p->precision != CHAN_FP64) return fail(CHAN_E_ARG, "precision mode %d is not available", p->precision);
+  if (p->cpi && p->cpi_type != 0 && p->cpi_type != 1) return fail(CHAN_E_CPITYPE, "Wrong selection of CPI_Type (%d)", p->cpi_type);
+  if (!(p->alfa0 > 0) || !(p->beta0 > 0) || !(p->ni > 0) || !(p->ymax > p->ymin) || !(p->a > 0))
+    return fail(CHAN_E_ARG, "alfa0, beta0, ni, a must be positive and ymax > ymin");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CHAN_E_NODEV, "no HIP device");
+  chan_ctx *c = new chan_ctx();
+  c->prm = *p;
+  setup_build(c->hs, p->nx, p->ny, p->nz, p->alfa0, p->beta0, p->a, p->ymin, p->ymax);
+  const HostSetup &h = c->hs;
+  if (!fft_size_supported(h.nzd) || !fft_size_supported(2 * h.nxd) || h.nzd % ZB != 0) {
+    int rc = fail(CHAN_E_SIZE, "transform sizes nzd=%d, 2nxd=%d are not supported", h.nzd, 2 * h.nxd);
+    setup_free(c->hs);
+    delete c;
+    return rc;
+  }
+  Geom &g = c->g;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.nz = p->nz;
+  g.Nx = p->nx + 1;
+  g.Nz = 2 * p->nz + 1;
+  g.Ny = p->ny + 3;
+  g.nxd = h.nxd;
+  g.nzd = h.nzd;
+  g.npair = g.Nx * g.Nz;
+  g.ps = ((size_t)g.npair + 63) / 64 * 64;
+  g.alfa0 = p->alfa0;
+  g.beta0 = p->beta0;
+  c->time = p->time;
+  c->deltat = p->deltat;
+
+  int rc = 0;
+  auto bail = [&](int code) {
+    chan_destroy(c);
+    return code;
+  };
+  if (hipStreamCreate(&c->st) != hipSuccess) return bail(fail(CHAN_E_STATE, "hipStreamCreate failed"));
+  const size_t plane = g.ps, NyP = (size_t)g.Ny * plane;
+  const size_t slab = (size_t)g.nzd * g.Nx;  // one component of one plane in physical z
+  if ((rc = dalloc(&c->V, 3 * NyP))) return bail(rc);
+  if ((rc = dalloc(&c->old, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
+  if ((rc = dalloc(&c->rhs, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
+  if ((rc = dalloc(&c->L, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
+  if ((rc = dalloc(&c->ucor, (size_t)g.Ny))) return bail(rc);
+  if ((rc = dalloc(&c->col00, 3 * (size_t)g.Ny))) return bail(rc);
+  if ((rc = dalloc(&c->A, 3 * (size_t)g.Ny * slab))) return bail(rc);
+  if ((rc = dalloc(&c->B, 6 * (size_t)g.Ny * slab))) return bail(rc);
+  if ((rc = dalloc(&c->P, 6 * NyP))) return bail(rc);
+  if ((rc = dalloc(&c->twz, (size_t)g.nzd))) return bail(rc);
+  if ((rc = dalloc(&c->twx, (size_t)2 * g.nxd))) return bail(rc);
+  if ((rc = dalloc(&c->der, (size_t)(g.ny - 1) * 20))) return bail(rc);
+  if ((rc = dalloc(&c->d0fac, (size_t)(g.ny + 1) * 5))) return bail(rc);
+  if ((rc = dalloc(&c->y, (size_t)g.Ny))) return bail(rc);
+  if ((rc = dalloc(&c->dy, (size_t)g.ny + 1))) return bail(rc);
+  if ((rc = dalloc(&c->sc, 1))) return bail(rc);
+  c->ncflpart = (size_t)(g.nzd / ZB) * g.Ny;
+  if ((rc = dalloc(&c->cflpart, c->ncflpart))) return bail(rc);
+
+  std::vector<cx<double>> tw;
+  twiddles(tw, g.nzd);
+  if (hipMemcpy(c->twz, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
+  twiddles(tw, 2 * g.nxd);
+  if (hipMemcpy(c->twx, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
+  if (hipMemcpy(c->der, h.der, sizeof(double) * 20 * (g.ny - 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d0fac, h.d0fac, sizeof(double) * 5 * (g.ny + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->y, h.y, sizeof(double) * g.Ny, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->dy, h.dy, sizeof(double) * (g.ny + 1), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(CHAN_E_STATE, "upload of the set-up failed"));
+  DevScalars s0;
+  std::memset(&s0, 0, sizeof s0);
+  s0.meanpx = p->meanpx;
+  s0.meanpz = p->meanpz;
+  if (hipMemcpy(c->sc, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
+  // stored explicit terms start at zero (ODE3 = 0 in the first substep)
+  if (hipMemset(c->old, 0, sizeof(cx<double>) * 2 * (g.ny - 1) * plane) != hipSuccess ||
+      hipMemset(c->V, 0, sizeof(cx<double>) * 3 * NyP) != hipSuccess ||
+      hipMemset(c->ucor, 0, sizeof(cx<double>) * g.Ny) != hipSuccess)
+    return bail(fail(CHAN_E_STATE, "hipMemset failed"));
+  *out = c;
+  return 0;
+}
+
+int chan_destroy(chan_ctx *c) {
+  if (!c) return 0;
+  if (c->st) hipStreamSynchronize(c->st);
+  for (auto &r : c->pend) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) hipEventDestroy(e);
+  void *bufs[] = {c->V, c->old, c->rhs, c->L, c->ucor, c->col00, c->A, c->B, c->P, c->twz, c->twx,
+                  c->der, c->d0fac, c->y, c->dy, c->sc, c->cflpart};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  if (c->st) hipStreamDestroy(c->st);
+  setup_free(c->hs);
+  delete c;
+  return 0;
+}
+
+int chan_sizes(const chan_ctx *c, int32_t s[5]) {
+  if (!c || !s) return fail(CHAN_E_ARG, "null argument");
+  s[0] = c->g.nx;
+  s[1] = c->g.ny;
+  s[2] = c->g.nz;
+  s[3] = c->g.nxd;
+  s[4] = c->g.nzd;
+  return 0;
+}
+
+int chan_get_setup(const chan_ctx *c, double *y, double *der, double *bnd) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  if (y) std::memcpy(y, c->hs.y, sizeof(double) * c->g.Ny);
+  if (der) std::memcpy(der, c->hs.der, sizeof(double) * 20 * (c->g.ny - 1));
+  if (bnd) std::memcpy(bnd, c->hs.bnd, sizeof(double) * 50);
+  return 0;
+}
+
+int chan_setup_host(const chan_params *p, int32_t sizes[5], double *y, double *der, double *bnd, double *bc,
+                    double *d0fac) {
+  if (!p) return fail(CHAN_E_ARG, "null argument");
+  if (p->nx < 1 || p->nz < 1 || p->ny < 8) return fail(CHAN_E_ARG, "need nx >= 1, nz >= 1, ny >= 8");
+  HostSetup h;
+  setup_build(h, p->nx, p->ny, p->nz, p->alfa0, p->beta0, p->a, p->ymin, p->ymax);
+  if (sizes) {
+    sizes[0] = p->nx;
+    sizes[1] = p->ny;
+    sizes[2] = p->nz;
+    sizes[3] = h.nxd;
+    sizes[4] = h.nzd;
+  }
+  if (y) std::memcpy(y, h.y, sizeof(double) * (p->ny + 3));
+  if (der) std::memcpy(der, h.der, sizeof(double) * 20 * (p->ny - 1));
+  if (bnd) std::memcpy(bnd, h.bnd, sizeof(double) * 50);
+  if (bc) std::memcpy(bc, h.bc, sizeof(double) * 40);
+  if (d0fac) std::memcpy(d0fac, h.d0fac, sizeof(double) * 5 * (p->ny + 1));
+  setup_free(h);
+  return 0;
+}
+
+int chan_set_field(chan_ctx *c, const double *V) {
+  if (!c || !V) return fail(CHAN_E_ARG, "null argument");
+  const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
+  HIPCHK(hipMemcpyAsync(c->B, V, n * sizeof(cx<double>), hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_to_device_layout(c->st, c->B, c->V, c->g));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->field_set = true;
+  return 0;
+}
+
+int chan_get_field(chan_ctx *c, double *V) {
+  if (!c || !V) return fail(CHAN_E_ARG, "null argument");
+  const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
+  HIPCHK(launch_to_host_layout(c->st, c->V, c->B, c->g));
+  HIPCHK(hipMemcpyAsync(V, c->B, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int chan_get_rhs(chan_ctx *c, double *R) {
+  if (!c || !R) return fail(CHAN_E_ARG, "null argument");
+  // rhs is [2][ny-1][ps]: the same transpose as the field with ny-1 rows
+  Geom g = c->g;
+  g.Ny = g.ny - 1;
+  cx<double> *stage = c->B;
+  for (int k = 0; k < 2; k++) {
+    Geom g1 = g;
+    // one component at a time: the transpose kernel runs over 3 components,
+    // so reuse it with a single-component view
+    dim3 grid((unsigned)((g1.npair + 31) / 32), (unsigned)((g1.Ny + 31) / 32), 1);
+    hipLaunchKernelGGL(k_to_host, grid, dim3(32, 8), 0, c->st, c->rhs + (size_t)k * g1.Ny * g1.ps,
+                       stage + (size_t)k * g1.npair * g1.Ny, g1);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(R, stage, sizeof(cx<double>) * 2 * (size_t)g.npair * g.Ny, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int chan_get_scalars(chan_ctx *c, chan_scalars *s) {
+  if (!c || !s) return fail(CHAN_E_ARG, "null argument");
+  DevScalars d;
+  HIPCHK(hipMemcpyAsync(&d, c->sc, sizeof d, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  s->time = c->time;
+  s->deltat = c->deltat;
+  s->meanpx = d.meanpx;
+  s->meanpz = d.meanpz;
+  s->corrpx = d.corrpx;
+  s->corrpz = d.corrpz;
+  s->fr[0] = d.fr[0];
+  s->fr[1] = d.fr[1];
+  s->fr[2] = d.fr[2];
+  s->cfl = cfl_value(d);
+  return 0;
+}
+
+int chan_set_scalars(chan_ctx *c, const chan_scalars *s) {
+  if (!c || !s) return fail(CHAN_E_ARG, "null argument");
+  DevScalars d;
+  std::memset(&d, 0, sizeof d);
+  d.meanpx = s->meanpx;
+  d.meanpz = s->meanpz;
+  d.corrpx = s->corrpx;
+  d.corrpz = s->corrpz;
+  d.fr[0] = s->fr[0];
+  d.fr[1] = s->fr[1];
+  d.fr[2] = s->fr[2];
+  const double cf = s->cfl < 0 ? 0.0 : s->cfl;
+  std::memcpy(&d.cfl_bits, &cf, sizeof cf);
+  c->time = s->time;
+  c->deltat = s->deltat;
+  HIPCHK(hipMemcpyAsync(c->sc, &d, sizeof d, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int products(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
+  const Geom &g = c->g;
+  const HostSetup &h = c->hs;
+  int rc;
+  if ((rc = timed(c, "zinv", [&] { return launch_zinv<double>(g.nzd, c->st, c->V, c->A, c->twz, g, plane0, nplanes); }))) return rc;
+  if ((rc = timed(c, prods ? "xpass" : "xpass_cfl", [&] {
+         return launch_xpass<double>(2 * g.nxd, c->st, c->A, c->B, c->twx, g, plane0, nplanes, prods, cfl, c->dy, h.dx,
+                                     h.dz, h.factor, c->cflpart);
+       })))
+    return rc;
+  if (prods)
+    if ((rc = timed(c, "zfwd", [&] { return launch_zfwd<double>(g.nzd, c->st, c->B, c->P, c->twz, g, plane0, nplanes); }))) return rc;
+  if (cfl) {
+    const int n = (g.nzd / ZB) * nplanes;
+    if ((rc = timed(c, "cfl_reduce", [&] { return launch_cfl_reduce(c->st, c->cflpart, n, c->sc); }))) return rc;
+  }
+  return 0;
+}
+
+int chan_buildrhs(chan_ctx *c, const double ode[3], int32_t compute_cfl) {
+  if (!c || !ode) return fail(CHAN_E_ARG, "null argument");
+  if (!c->field_set) return fail(CHAN_E_STATE, "buildrhs before set_field");
+  const Geom &g = c->g;
+  int rc;
+  if ((rc = products(c, 0, g.Ny, true, compute_cfl ? 1 : 0))) return rc;
+  RhsArgs<double> a;
+  a.P = c->P;
+  a.V = c->V;
+  a.old = c->old;
+  a.rhs = c->rhs;
+  a.der = c->der;
+  a.sc = c->sc;
+  a.g = g;
+  a.ode0 = ode[0];
+  a.ode1 = ode[1];
+  a.ode2 = ode[2];
+  a.dt = c->deltat;
+  a.ni = c->prm.ni;
+  a.chunk = 48;
+  return timed(c, "rhs", [&] { return launch_rhs<double>(c->st, a); });
+}
+
+int chan_linsolve(chan_ctx *c, double lambda) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  if (!c->field_set) return fail(CHAN_E_STATE, "linsolve before set_field");
+  const Geom &g = c->g;
+  int rc;
+  SolveArgs s;
+  s.V = c->V;
+  s.rhs = c->rhs;
+  s.L = c->L;
+  s.ucor = c->ucor;
+  s.der = c->der;
+  std::memcpy(s.bc, c->hs.bc, sizeof s.bc);
+  s.g = g;
+  s.lambda = lambda;
+  s.ni = c->prm.ni;
+  s.u0 = c->prm.u0;
+  s.uN = c->prm.uN;
+  if ((rc = timed(c, "solve_ve", [&] { return launch_solve(c->st, s); }))) return rc;
+  MeanArgs m;
+  m.V = c->V;
+  m.ucor = c->ucor;
+  m.y = c->y;
+  m.sc = c->sc;
+  std::memcpy(m.bc, c->hs.bc, sizeof m.bc);
+  m.g = g;
+  m.meanflowx = c->prm.meanflowx;
+  m.meanflowz = c->prm.meanflowz;
+  m.gamma = c->prm.gamma;
+  m.ni = c->prm.ni;
+  m.cpi = c->prm.cpi;
+  m.cpi_type = c->prm.cpi_type;
+  if ((rc = timed(c, "mean", [&] { return launch_mean(c->st, m); }))) return rc;
+  UwArgs u;
+  u.V = c->V;
+  u.der = c->der;
+  u.d0fac = c->d0fac;
+  std::memcpy(u.bnd, c->hs.bnd, sizeof u.bnd);
+  u.g = g;
+  return timed(c, "uw", [&] { return launch_uw(c->st, u); });
+}
+
+int chan_initial(chan_ctx *c) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  if (!c->field_set) return fail(CHAN_E_STATE, "initial before set_field");
+  if (c->deltat == 0) c->deltat = 1.0;
+  const Geom &g = c->g;
+  int rc;
+  // interior planes iy = 1..ny-1 are plane indices 2..ny
+  if ((rc = products(c, 2, g.ny - 1, false, 1))) return rc;
+  MeanArgs m;
+  std::memset(&m, 0, sizeof m);
+  m.V = c->V;
+  m.y = c->y;
+  m.sc = c->sc;
+  m.g = g;
+  m.gamma = c->prm.gamma;
+  m.ni = c->prm.ni;
+  m.cpi = c->prm.cpi;
+  m.cpi_type = c->prm.cpi_type;
+  return timed(c, "flowrate", [&] { return launch_flowrate(c->st, m); });
+}
+
+int chan_outstats(chan_ctx *c, double row[11]) {
+  if (!c || !row) return fail(CHAN_E_ARG, "null argument");
+  const Geom &g = c->g;
+  std::vector<cx<double>> col(3 * (size_t)g.Ny);
+  DevScalars d;
+  hipLaunchKernelGGL(k_gather00, dim3((g.Ny + 127) / 128), dim3(128), 0, c->st, c->V, c->col00, g);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(col.data(), c->col00, col.size() * sizeof(col[0]), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&d, c->sc, sizeof d, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const double rg = cfl_value(d);
+  if (!std::isfinite(rg)) return fail(CHAN_E_NOTFINITE, "CFL maximum is not finite (%g): the field holds NaN or Inf", rg);
+  // reset the running maximum (cfl=0 in outstats)
+  unsigned long long zero = 0;
+  HIPCHK(hipMemcpyAsync(&c->sc->cfl_bits, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (c->prm.cflmax > 0) c->deltat = c->prm.cflmax / rg;
+  const double *b140 = c->hs.bnd[B_D140], *b14n = c->hs.bnd[B_D14N];
+  const int ny = g.ny;
+  auto re = [&](int comp, int iy) { return col[(size_t)comp * g.Ny + iy + 1].x; };
+  double d11 = 0, d21 = 0, d12 = 0, d22 = 0;
+  for (int j = 0; j < 5; j++) {
+    d12 += b14n[j] * re(0, ny - 3 + j);
+    d22 += b14n[j] * re(2, ny - 3 + j);
+  }
+  d12 = -d12;
+  d22 = -d22;
+  for (int j = 0; j < 5; j++) {
+    d11 += b140[j] * re(0, -1 + j);
+    d21 += b140[j] * re(2, -1 + j);
+  }
+  row[0] = c->time;
+  row[1] = d11;
+  row[2] = d12;
+  row[3] = d21;
+  row[4] = d22;
+  row[5] = d.fr[0] + d.corrpx * d.fr[2];
+  row[6] = d.meanpx + d.corrpx;
+  row[7] = d.fr[1] + d.corrpz * d.fr[2];
+  row[8] = d.meanpz + d.corrpz;
+  row[9] = rg * c->deltat;
+  row[10] = c->deltat;
+  return 0;
+}
+
+int chan_step(chan_ctx *c, double row[11]) {
+  if (!c || !row) return fail(CHAN_E_ARG, "null argument");
+  int rc;
+  const double *ode[3] = {RK1, RK2, RK3};
+  for (int k = 0; k < 3; k++) {
+    c->time = c->time + 2.0 / ode[k][0] * c->deltat;
+    if ((rc = chan_buildrhs(c, ode[k], k == 2))) return rc;
+    if ((rc = chan_linsolve(c, ode[k][0] / c->deltat))) return rc;
+  }
+  return chan_outstats(c, row);
+}
+
+int chan_steps(chan_ctx *c, int32_t n, double *rows) {
+  if (!c || (n > 0 && !rows)) return fail(CHAN_E_ARG, "null argument");
+  for (int32_t i = 0; i < n; i++) {
+    int rc = chan_step(c, rows + 11 * (size_t)i);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int chan_timing(chan_ctx *c, int32_t enable) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  if (!enable) {
+    int rc = harvest(c);
+    if (rc) return rc;
+  }
+  c->timing = enable != 0;
+  if (enable) {
+    int rc = harvest(c);
+    if (rc) return rc;
+    for (size_t i = 0; i < c->names.size(); i++) {
+      c->counts[i] = 0;
+      c->ms[i] = 0;
+    }
+  }
+  return 0;
+}
+
+int chan_kernel_times(chan_ctx *c, int32_t cap, char *names, int64_t *counts, double *total_ms, int32_t *n) {
+  if (!c || !n) return fail(CHAN_E_ARG, "null argument");
+  int rc = harvest(c);
+  if (rc) return rc;
+  *n = (int32_t)c->names.size();
+  for (int32_t i = 0; i < *n && i < cap; i++) {
+    if (names) {
+      std::strncpy(names + 32 * i, c->names[i].c_str(), 31);
+      names[32 * i + 31] = 0;
+    }
+    if (counts) counts[i] = c->counts[i];
+    if (total_ms) total_ms[i] = c->ms[i];
+  }
+  return 0;
+}
+
+int chan_sync(chan_ctx *c) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout) {
+  if (!inout || batch < 1 || (sign != 1 && sign != -1)) return fail(CHAN_E_ARG, "bad argument");
+  if (!fft_size_supported(n)) return fail(CHAN_E_SIZE, "size %d not supported", n);
+  std::vector<cx<double>> tw;
+  twiddles(tw, n);
+  cx<double> *d = nullptr, *t = nullptr;
+  const size_t bytes = sizeof(cx<double>) * (size_t)n * batch;
+  HIPCHK(hipMalloc(&d, bytes));
+  hipError_t e = hipMalloc(&t, sizeof(cx<double>) * n);
+  if (e == hipSuccess) e = hipMemcpy(d, inout, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_fft_test(n, sign, 0, d, t, batch);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(inout, d, bytes, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (t) hipFree(t);
+  if (e != hipSuccess) return fail((int)e, "fft selftest: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// chan_internal.hpp -- state and kernel interfaces shared by the library's
+// translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "fft.hpp"
+
+namespace chan {
+
+// Intermediate layouts of the transforms use blocks of ZB consecutive
+// physical-z points so that the z-transform kernels read and write ZB*16
+// contiguous bytes per x-mode.
+constexpr int ZB = 4;
+
+// Set-up products computed on the host (setup.cpp) and mirrored to device.
+struct HostSetup {
+  int nx, ny, nz, nxd, nzd;
+  double PI, dx, dz, factor;
+  double *y;      // [ny+3], y[iy+1]
+  double *dy;     // [ny+1], dy[iy] for iy=1..ny-1
+  double *der;    // [(ny-1)][4][5]: d0,d1,d2,d4 of rows iy=1..ny-1
+  double *d0fac;  // [(ny+1)][5]: D0 band after the UL factorisation
+  double bnd[10][5];  // d040,d140,d240,d14m1,d24m1,d04n,d14n,d24n,d14np1,d24np1
+  double bc[8][5];    // v0bc,v0m1bc,eta0bc,eta0m1bc,vnbc,vnp1bc,etanbc,etanp1bc
+};
+enum { B_D040, B_D140, B_D240, B_D14M1, B_D24M1, B_D04N, B_D14N, B_D24N, B_D14NP1, B_D24NP1 };
+enum { BC_V0, BC_V0M1, BC_ETA0, BC_ETA0M1, BC_VN, BC_VNP1, BC_ETAN, BC_ETANP1 };
+
+int fftfit(int i);
+void setup_build(HostSetup &s, int nx, int ny, int nz, double alfa0, double beta0, double a, double ymin, double ymax);
+void setup_free(HostSetup &s);
+
+// Geometry passed by value to kernels.
+struct Geom {
+  int nx, ny, nz;   // dns.in sizes
+  int Nx, Nz, Ny;   // nx+1, 2nz+1, ny+3
+  int nxd, nzd;     // dealiased sizes, physical grid 2nxd x nzd
+  int npair;        // Nx*Nz wavenumber pairs, pair p = ix*Nz + (iz+nz)
+  size_t ps;        // plane stride (pairs, padded)
+  double alfa0, beta0;
+};
+
+// Device copies of the set-up.
+struct DevSetup {
+  const double *der;    // [(ny-1)][4][5]
+  const double *d0fac;  // [(ny+1)][5]
+  const double *y;      // [ny+3]
+  const double *dy;     // [ny+1]
+  double bnd[10][5];
+  double bc[8][5];
+};
+
+// Scalars that live on the device between kernels (the (0,0) mode writes
+// them in linsolve, buildrhs reads meanpx/meanpz).
+struct DevScalars {
+  double meanpx, meanpz, corrpx, corrpz;
+  double fr[3];
+  unsigned long long cfl_bits;  // running CFL maximum as the bits of a non-negative double
+  double pad[8];
+};
+
+// ---------------------------------------------------------- kernel args --
+template <typename R>
+struct RhsArgs {
+  const cx<R> *P;        // [6][Ny][ps] products
+  const cx<double> *V;   // [3][Ny][ps]
+  cx<double> *old;       // [2][ny-1][ps] previous explicit terms (eta, d2v)
+  cx<double> *rhs;       // [2][ny-1][ps]
+  const double *der;     // [(ny-1)][4][5]
+  const DevScalars *sc;  // meanpx, meanpz
+  Geom g;
+  double ode0, ode1, ode2, dt, ni;
+  int chunk;
+};
+
+struct SolveArgs {
+  cx<double> *V;           // [3][Ny][ps]; components 1,2 receive eta and v
+  const cx<double> *rhs;   // [2][ny-1][ps]
+  double2 *L;              // [2][ny-1][ps] lower factors of the UL decomposition
+  cx<double> *ucor;        // [Ny] unit-forcing response of the (0,0) eta system
+  const double *der;
+  double bc[8][5];
+  Geom g;
+  double lambda, ni, u0, uN;
+};
+
+struct MeanArgs {
+  cx<double> *V;
+  const cx<double> *ucor;
+  const double *y;
+  DevScalars *sc;
+  double bc[8][5];
+  Geom g;
+  double meanflowx, meanflowz, gamma, ni;
+  int cpi, cpi_type;
+};
+
+struct UwArgs {
+  cx<double> *V;
+  const double *der;
+  const double *d0fac;
+  double bnd[10][5];
+  Geom g;
+};
+
+// ----------------------------------------------------------- launchers --
+int fft_size_supported(int N);
+template <typename R>
+hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, const cx<R> *tw, const Geom &g, int plane0,
+                       int nplanes);
+template <typename R>
+size_t xpass_smem(int M, const Geom &g);
+template <typename R>
+hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
+                        int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
+                        double *cfl_part);
+template <typename R>
+hipError_t launch_zfwd(int NZ, hipStream_t st, const cx<R> *B, cx<R> *P, const cx<R> *tw, const Geom &g, int plane0,
+                       int nplanes);
+hipError_t launch_fft_test(int N, int sign, hipStream_t st, cx<double> *data, const cx<double> *tw, int batch);
+template <typename R>
+hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a);
+hipError_t launch_solve(hipStream_t st, const SolveArgs &a);
+hipError_t launch_mean(hipStream_t st, const MeanArgs &a);
+hipError_t launch_uw(hipStream_t st, const UwArgs &a);
+hipError_t launch_flowrate(hipStream_t st, const MeanArgs &a);
+hipError_t launch_cfl_reduce(hipStream_t st, const double *part, int n, DevScalars *sc);
+hipError_t launch_to_device_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g);
+hipError_t launch_to_host_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g);
+
+}  // namespace chan

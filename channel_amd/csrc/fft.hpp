@@ -1,0 +1,189 @@
+// fft.hpp -- batched mixed-radix (2,3,4) Stockham FFT for CDNA4, one team of
+// T threads per transform, data resident in registers between radix passes
+// and exchanged through LDS.
+//
+// Slot convention: a team of T threads transforms N = E*T points; thread t
+// holds "slot" s (0 <= s < E) = element t + T*s both on input and on output,
+// so loads from and stores to global memory by consecutive threads are
+// contiguous.  Sizes N = 3*2^k use E = 12, N = 2^k use E = 16; every radix
+// divides E, so each pass maps whole butterflies onto each thread.
+//
+// Conventions match FFTW's (ffts.f90:70-75): sign -1 forward
+// X_k = sum_j x_j e^{-2 pi i jk/N}, sign +1 backward, both unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace chan {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+  R x, y;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) {
+  return cx<R>{a, b};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> add(cx<R> a, cx<R> b) {
+  return cx<R>{a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> sub(cx<R> a, cx<R> b) {
+  return cx<R>{a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> mul(cx<R> a, cx<R> b) {
+  return cx<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> scl(R s, cx<R> a) {
+  return cx<R>{s * a.x, s * a.y};
+}
+template <typename R>
+__host__ __device__ __forceinline__ cx<R> conj(cx<R> a) {
+  return cx<R>{a.x, -a.y};
+}
+
+namespace fft {
+
+template <int N>
+struct Plan {
+  static constexpr bool pow2 = (N & (N - 1)) == 0;
+  static constexpr int E = pow2 ? 16 : 12;
+  static constexpr int T = N / E;
+  static_assert(T >= 1 && T * E == N, "unsupported FFT length");
+  static constexpr int base() { return pow2 ? N : N / 3; }
+  static constexpr int n4() {
+    int r = base(), n = 0;
+    while (r % 4 == 0) {
+      r /= 4;
+      n++;
+    }
+    return n;
+  }
+  static constexpr bool has2() {
+    int r = base();
+    while (r % 4 == 0) r /= 4;
+    return r == 2;
+  }
+  static constexpr int npass() { return n4() + (has2() ? 1 : 0) + (pow2 ? 0 : 1); }
+  static constexpr int radix(int p) { return p < n4() ? 4 : ((p == n4() && has2()) ? 2 : 3); }
+};
+
+// team-level LDS synchronisation: a team within one wavefront only needs the
+// wave's LDS operations ordered; larger teams use the workgroup barrier.
+template <int T>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (T <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int RAD, int S, typename R>
+__device__ __forceinline__ void dft(cx<R> *a) {
+  if constexpr (RAD == 2) {
+    cx<R> t = sub(a[0], a[1]);
+    a[0] = add(a[0], a[1]);
+    a[1] = t;
+  } else if constexpr (RAD == 4) {
+    cx<R> t0 = add(a[0], a[2]), t1 = sub(a[0], a[2]);
+    cx<R> t2 = add(a[1], a[3]), t3 = sub(a[1], a[3]);
+    a[0] = add(t0, t2);
+    a[2] = sub(t0, t2);
+    if constexpr (S < 0) {  // y1 = t1 - i t3, y3 = t1 + i t3
+      a[1] = cx<R>{t1.x + t3.y, t1.y - t3.x};
+      a[3] = cx<R>{t1.x - t3.y, t1.y + t3.x};
+    } else {  // y1 = t1 + i t3, y3 = t1 - i t3
+      a[1] = cx<R>{t1.x - t3.y, t1.y + t3.x};
+      a[3] = cx<R>{t1.x + t3.y, t1.y - t3.x};
+    }
+  } else {
+    static_assert(RAD == 3, "radix");
+    const R s = (S < 0 ? R(-1) : R(1)) * R(0.86602540378443864676372317075293618);
+    cx<R> t1 = add(a[1], a[2]);
+    cx<R> d = sub(a[1], a[2]);
+    cx<R> t2 = cx<R>{a[0].x - R(0.5) * t1.x, a[0].y - R(0.5) * t1.y};
+    a[0] = add(a[0], t1);
+    a[1] = cx<R>{t2.x - s * d.y, t2.y + s * d.x};
+    a[2] = cx<R>{t2.x + s * d.y, t2.y - s * d.x};
+  }
+}
+
+// One Stockham pass (decimation in time, autosort) on the thread's E values
+// held in "read order" v[m*RAD + r] = in[j + r*N/RAD], j = t + m*T; then the
+// exchange through LDS into the read order of the next pass.  LDS holds N
+// reals per team: real and imaginary parts travel in two rounds.
+template <int N, int S, int P, int NS, typename R>
+__device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
+  using PL = Plan<N>;
+  constexpr int E = PL::E, T = PL::T, RAD = PL::radix(P), NB = E / RAD;
+  constexpr int TWS = N / (NS * RAD);
+#pragma unroll
+  for (int m = 0; m < NB; m++) {
+    if constexpr (NS > 1) {
+      const int j = t + m * T;
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < RAD; r++) {
+        cx<R> w = tw[r * k * TWS];
+        if constexpr (S > 0) w.y = -w.y;
+        v[m * RAD + r] = mul(v[m * RAD + r], w);
+      }
+    }
+    dft<RAD, S>(&v[m * RAD]);
+  }
+  if constexpr (P + 1 < PL::npass()) {
+    constexpr int R2 = PL::radix(P + 1), NB2 = E / R2;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+#pragma unroll
+      for (int m = 0; m < NB; m++) {
+        const int j = t + m * T;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * RAD + k;
+#pragma unroll
+        for (int q = 0; q < RAD; q++) lds[base + q * NS] = half ? v[m * RAD + q].y : v[m * RAD + q].x;
+      }
+      team_sync<T>();
+#pragma unroll
+      for (int m = 0; m < NB2; m++)
+#pragma unroll
+        for (int r = 0; r < R2; r++) {
+          R val = lds[t + m * T + r * (N / R2)];
+          if (half)
+            v[m * R2 + r].y = val;
+          else
+            v[m * R2 + r].x = val;
+        }
+      team_sync<T>();
+    }
+    run_pass<N, S, P + 1, NS * RAD, R>(v, lds, tw, t);
+  }
+}
+
+// Transform the team's N points held in slots e[s] = x[t + T*s]; on return
+// e[s] = X[t + T*s].  lds: N reals private to the team.  tw: N twiddles
+// tw[e] = exp(-2 pi i e/N).
+template <int N, int S, typename R>
+__device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
+  using PL = Plan<N>;
+  constexpr int E = PL::E, R1 = PL::radix(0), RL = PL::radix(PL::npass() - 1);
+  cx<R> v[E];
+#pragma unroll
+  for (int m = 0; m < E / R1; m++)
+#pragma unroll
+    for (int r = 0; r < R1; r++) v[m * R1 + r] = e[m + r * (E / R1)];
+  run_pass<N, S, 0, 1, R>(v, lds, tw, t);
+#pragma unroll
+  for (int m = 0; m < E / RL; m++)
+#pragma unroll
+    for (int q = 0; q < RL; q++) e[m + q * (E / RL)] = v[m * RL + q];
+}
+
+}  // namespace fft
+}  // namespace chan

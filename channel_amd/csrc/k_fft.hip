@@ -1,0 +1,371 @@
+// k_fft.hip -- the convolution products of one RK3 substep as three kernels
+// over all y planes (convolutions, dnsdata.f90:487-602):
+//
+//   K1 k_zinv : V(iy,iz,ix,c) -> zero-pad z to nzd, backward c2c FFT in z
+//               (IFT, ffts.f90:93-96)  -> A[plane][c][zb][ix][ZB]
+//   K2 k_xpass: per physical-z row: Hermitian c2r in x (RFT, zero modes
+//               nx+1..nxd), CFL maximum, the six products times `factor`,
+//               r2c in x keeping modes 0..nx (HFT)      -> B[plane][q][zb][ix][ZB]
+//   K3 k_zfwd : forward c2c FFT in z, keep modes -nz..nz (FFT + izd)
+//                                                       -> P[q][plane][pair]
+//
+// The reference's pencil transposes (zTOx / xTOz, mpi_transpose.f90:50-117)
+// are the layout changes between K1/K2 and K2/K3: both sides move ZB*16
+// contiguous bytes per x mode.  Two real x-transforms share one complex FFT
+// (u+iv on the way in, product pairs on the way out).
+#include "chan_internal.hpp"
+
+namespace chan {
+
+// ------------------------------------------------------------------- K1 --
+template <int NZ, typename R>
+__global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, cx<R> *__restrict__ A,
+                                              const cx<R> *__restrict__ tw, Geom g, int plane0) {
+  using PL = fft::Plan<NZ>;
+  constexpr int T = PL::T, E = PL::E, G = 256 / T;
+  __shared__ R lds[G * NZ];
+  const int team = threadIdx.x / T, t = threadIdx.x % T;
+  const int nxg = (g.Nx + G - 1) / G;
+  int b = blockIdx.x;
+  const int xg = b % nxg;
+  b /= nxg;
+  const int c = b % 3, pl = b / 3;
+  const int ix = xg * G + team;
+  const bool act = ix < g.Nx;
+  const cx<double> *col = V + ((size_t)c * g.Ny + (plane0 + pl)) * g.ps + (size_t)(act ? ix : 0) * g.Nz;
+  cx<R> e[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) {
+    const int z = t + T * s;
+    const int iz = z <= g.nz ? z + g.nz : (z >= NZ - g.nz ? z - NZ + g.nz : -1);
+    cx<double> v = {0.0, 0.0};
+    if (act && iz >= 0) v = col[iz];
+    e[s] = cx<R>{(R)v.x, (R)v.y};
+  }
+  fft::fft_slots<NZ, +1>(e, lds + team * NZ, tw, t);
+  if (act) {
+    cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int z = t + T * s;
+      dst[((size_t)(z / ZB) * g.Nx + ix) * ZB + (z % ZB)] = e[s];
+    }
+  }
+}
+
+// ------------------------------------------------------------------- K2 --
+template <int NT>
+__device__ __forceinline__ void block_max_store(double m, double *red, double *out) {
+  red[threadIdx.x] = m;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double o = red[threadIdx.x + s];
+      const double c = red[threadIdx.x];
+      red[threadIdx.x] = (o > c || o != o) ? o : c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+template <int M, typename R, bool PRODUCTS>
+__global__ __launch_bounds__(ZB *fft::Plan<M>::T) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
+                                                                const cx<R> *__restrict__ tw, Geom g, int plane0,
+                                                                int cfl_on, const double *__restrict__ dy, double dx,
+                                                                double dz, R factor, double *__restrict__ cfl_part) {
+  using PL = fft::Plan<M>;
+  constexpr int T = PL::T, E = PL::E, NT = ZB * T;
+  constexpr int MO = E / 2 + 1;  // x modes per thread: ceil(Nx/T) <= E/2
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[NT];
+  const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
+  cx<R> *stage = reinterpret_cast<cx<R> *>(smem);                          // [3][ZB][Nx]
+  R *fl = reinterpret_cast<R *>(smem + sizeof(cx<R>) * 3 * ZB * Nx);        // [ZB][M]
+  const int tid = threadIdx.x, row = tid / T, t = tid % T;
+  const int zb = blockIdx.x % NZB, pl = blockIdx.x / NZB;
+  const int iy = plane0 + pl - 1;
+
+  // stage the ZB rows of u, v, w: A holds [ix][zi] runs of Nx*ZB values
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB;
+    for (int i = tid; i < Nx * ZB; i += NT) stage[(c * ZB + (i % ZB)) * Nx + i / ZB] = src[i];
+  }
+  __syncthreads();
+
+  // Hermitian extension to M points; Im of mode 0 is dropped as c2r does
+  cx<R> e1[E], e2[E];
+  {
+    const cx<R> *su = stage + (0 * ZB + row) * Nx, *sv = stage + (1 * ZB + row) * Nx, *sw = stage + (2 * ZB + row) * Nx;
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int k = t + T * s;
+      cx<R> U = {0, 0}, Vv = {0, 0}, W = {0, 0};
+      if (k <= nx) {
+        U = su[k];
+        Vv = sv[k];
+        W = sw[k];
+        if (k == 0) U.y = Vv.y = W.y = 0;
+      } else if (k >= M - nx) {
+        U = conj(su[M - k]);
+        Vv = conj(sv[M - k]);
+        W = conj(sw[M - k]);
+      }
+      e1[s] = cx<R>{U.x - Vv.y, U.y + Vv.x};
+      e2[s] = W;
+    }
+  }
+  __syncthreads();  // the stage area is reused for the outputs
+
+  R *fr = fl + row * M;
+  fft::fft_slots<M, +1>(e1, fr, tw, t);
+  fft::fft_slots<M, +1>(e2, fr, tw, t);
+  R u[E], v[E], w[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) {
+    u[s] = e1[s].x;
+    v[s] = e1[s].y;
+    w[s] = e2[s].x;
+  }
+
+  // CFL maximum over the interior planes (dnsdata.f90:552-556)
+  if (cfl_on && iy >= 1 && iy <= g.ny - 1) {
+    const double dyv = dy[iy];
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const double c = fabs((double)u[s]) / dx + fabs((double)v[s]) / dyv + fabs((double)w[s]) / dz;
+      m = (c > m || c != c) ? c : m;
+    }
+    block_max_store<NT>(m, red, cfl_part + blockIdx.x);
+  }
+
+  if constexpr (PRODUCTS) {
+    cx<R> *ost = stage;  // [2][ZB][Nx]
+#pragma unroll
+    for (int pp = 0; pp < 3; pp++) {
+      cx<R> e[E];
+#pragma unroll
+      for (int s = 0; s < E; s++) {
+        R a, b;
+        if (pp == 0) {
+          a = u[s] * u[s] * factor;
+          b = v[s] * v[s] * factor;
+        } else if (pp == 1) {
+          a = w[s] * w[s] * factor;
+          b = u[s] * v[s] * factor;
+        } else {
+          a = v[s] * w[s] * factor;
+          b = u[s] * w[s] * factor;
+        }
+        e[s] = cx<R>{a, b};
+      }
+      fft::fft_slots<M, -1>(e, fr, tw, t);
+      // split Z = FFT(a + ib) into A_k = (Z_k + conj Z_{M-k})/2 and
+      // B_k = (Z_k - conj Z_{M-k})/(2i) for k = 0..nx
+      R kx[MO], ky[MO], mx[MO], my[MO];
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+#pragma unroll
+        for (int s = 0; s < E; s++) fr[t + T * s] = half ? e[s].y : e[s].x;
+        fft::team_sync<T>();
+#pragma unroll
+        for (int m = 0; m < MO; m++) {
+          const int k = t + T * m;
+          if (k <= nx) {
+            const R zk = fr[k], zm = fr[k == 0 ? 0 : M - k];
+            if (half) {
+              ky[m] = zk;
+              my[m] = zm;
+            } else {
+              kx[m] = zk;
+              mx[m] = zm;
+            }
+          }
+        }
+        fft::team_sync<T>();
+      }
+#pragma unroll
+      for (int m = 0; m < MO; m++) {
+        const int k = t + T * m;
+        if (k <= nx) {
+          ost[(0 * ZB + row) * Nx + k] = cx<R>{(kx[m] + mx[m]) * R(0.5), (ky[m] - my[m]) * R(0.5)};
+          ost[(1 * ZB + row) * Nx + k] = cx<R>{(ky[m] + my[m]) * R(0.5), (mx[m] - kx[m]) * R(0.5)};
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        cx<R> *dst = B + ((size_t)(pl * 6 + 2 * pp + h) * NZB + zb) * Nx * ZB;
+        for (int i = tid; i < Nx * ZB; i += NT) dst[i] = ost[(h * ZB + (i % ZB)) * Nx + i / ZB];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------- K3 --
+template <int NZ, typename R>
+__global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R> *__restrict__ P,
+                                              const cx<R> *__restrict__ tw, Geom g, int plane0) {
+  using PL = fft::Plan<NZ>;
+  constexpr int T = PL::T, E = PL::E, G = 256 / T;
+  __shared__ R lds[G * NZ];
+  const int team = threadIdx.x / T, t = threadIdx.x % T;
+  const int nxg = (g.Nx + G - 1) / G;
+  int b = blockIdx.x;
+  const int xg = b % nxg;
+  b /= nxg;
+  const int q = b % 6, pl = b / 6;
+  const int ix = xg * G + team;
+  const bool act = ix < g.Nx;
+  const cx<R> *src = B + (size_t)(pl * 6 + q) * NZ * g.Nx;
+  cx<R> e[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) {
+    const int z = t + T * s;
+    e[s] = act ? src[((size_t)(z / ZB) * g.Nx + ix) * ZB + (z % ZB)] : cx<R>{0, 0};
+  }
+  fft::fft_slots<NZ, -1>(e, lds + team * NZ, tw, t);
+  if (act) {
+    cx<R> *dst = P + ((size_t)q * g.Ny + (plane0 + pl)) * g.ps + (size_t)ix * g.Nz;
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int k = t + T * s;
+      const int iz = k <= g.nz ? k + g.nz : (k >= NZ - g.nz ? k - NZ + g.nz : -1);
+      if (iz >= 0) dst[iz] = e[s];
+    }
+  }
+}
+
+// ------------------------------------------------------------- selftest --
+template <int N, int S>
+__global__ __launch_bounds__(256) void k_fft_test(cx<double> *__restrict__ data, const cx<double> *__restrict__ tw,
+                                                  int batch) {
+  using PL = fft::Plan<N>;
+  constexpr int T = PL::T, E = PL::E, G = 256 / T;
+  __shared__ double lds[G * N];
+  const int team = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blockIdx.x * G + team;
+  const bool act = row < batch;
+  cx<double> *x = data + (size_t)(act ? row : 0) * N;
+  cx<double> e[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) e[s] = act ? x[t + T * s] : cx<double>{0, 0};
+  fft::fft_slots<N, S>(e, lds + team * N, tw, t);
+  if (act)
+#pragma unroll
+    for (int s = 0; s < E; s++) x[t + T * s] = e[s];
+}
+
+// ------------------------------------------------------------ launchers --
+#define CHAN_FOR_SIZES(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+
+template <typename R>
+hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, const cx<R> *tw, const Geom &g, int plane0,
+                       int nplanes) {
+  switch (NZ) {
+#define X(N)                                                                              \
+  case N: {                                                                               \
+    constexpr int G = 256 / fft::Plan<N>::T;                                              \
+    dim3 grid((unsigned)(((g.Nx + G - 1) / G) * 3 * nplanes));                            \
+    hipLaunchKernelGGL((k_zinv<N, R>), grid, dim3(256), 0, st, V, A, tw, g, plane0);      \
+    return hipGetLastError();                                                             \
+  }
+    CHAN_FOR_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+size_t xpass_smem(int M, const Geom &g) {
+  return sizeof(cx<R>) * 3 * ZB * g.Nx + sizeof(R) * ZB * M;
+}
+
+template <typename R>
+hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
+                        int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
+                        double *cfl_part) {
+  dim3 grid((unsigned)((g.nzd / ZB) * nplanes));
+  const size_t sh = xpass_smem<R>(M, g);
+  switch (M) {
+#define X(N)                                                                                                      \
+  case N: {                                                                                                       \
+    dim3 blk(ZB * fft::Plan<N>::T);                                                                               \
+    if (products) {                                                                                               \
+      hipError_t ea = hipFuncSetAttribute((const void *)k_xpass<N, R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); \
+      if (ea != hipSuccess) return ea;                                                                            \
+      hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, sh, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor, \
+                         cfl_part);                                                                               \
+    } else {                                                                                                      \
+      hipError_t ea = hipFuncSetAttribute((const void *)k_xpass<N, R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); \
+      if (ea != hipSuccess) return ea;                                                                             \
+      hipLaunchKernelGGL((k_xpass<N, R, false>), grid, blk, sh, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor, \
+                         cfl_part);                                                                               \
+    }                                                                                                             \
+    return hipGetLastError();                                                                                     \
+  }
+    CHAN_FOR_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename R>
+hipError_t launch_zfwd(int NZ, hipStream_t st, const cx<R> *B, cx<R> *P, const cx<R> *tw, const Geom &g, int plane0,
+                       int nplanes) {
+  switch (NZ) {
+#define X(N)                                                                         \
+  case N: {                                                                          \
+    constexpr int G = 256 / fft::Plan<N>::T;                                         \
+    dim3 grid((unsigned)(((g.Nx + G - 1) / G) * 6 * nplanes));                       \
+    hipLaunchKernelGGL((k_zfwd<N, R>), grid, dim3(256), 0, st, B, P, tw, g, plane0); \
+    return hipGetLastError();                                                        \
+  }
+    CHAN_FOR_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fft_test(int N, int sign, hipStream_t st, cx<double> *data, const cx<double> *tw, int batch) {
+  switch (N) {
+#define X(NN)                                                                                      \
+  case NN: {                                                                                       \
+    constexpr int G = 256 / fft::Plan<NN>::T;                                                      \
+    dim3 grid((unsigned)((batch + G - 1) / G));                                                    \
+    if (sign < 0)                                                                                  \
+      hipLaunchKernelGGL((k_fft_test<NN, -1>), grid, dim3(256), 0, st, data, tw, batch);           \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_fft_test<NN, +1>), grid, dim3(256), 0, st, data, tw, batch);           \
+    return hipGetLastError();                                                                      \
+  }
+    CHAN_FOR_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+int fft_size_supported(int N) {
+  switch (N) {
+#define X(NN) case NN:
+    CHAN_FOR_SIZES(X)
+#undef X
+    return 1;
+  }
+  return 0;
+}
+
+template hipError_t launch_zinv<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
+                                        const Geom &, int, int);
+template hipError_t launch_xpass<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
+                                         const Geom &, int, int, bool, int, const double *, double, double, double,
+                                         double *);
+template hipError_t launch_zfwd<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
+                                        const Geom &, int, int);
+template size_t xpass_smem<double>(int, const Geom &);
+
+}  // namespace chan

@@ -1,0 +1,157 @@
+/*
+ * channel_hip.h -- C-ABI of the MI355X-native channel DNS time step.
+ *
+ * The reference has no plugin API: its per-step seam is the module-level
+ * subroutine interface of MODULE dnsdata, called from the time loop of
+ * PROGRAM channel.  Each entry point below replaces one of those routines and
+ * keeps its argument meaning and fail-stop behaviour; the library owns the
+ * device-resident state (the role of the module globals, dnsdata.f90:25-90).
+ *
+ * Conventions
+ *  - every function returns int: 0 = success, > 0 = HIP error code,
+ *    < 0 = invalid argument / state (see CHAN_E_*); chan_last_error() gives
+ *    the message.  A Fortran host prints it and STOPs, as the reference does.
+ *  - fields cross the boundary in the reference's own layout and type:
+ *    complex128 V(-1:ny+1, -nz:nz, 0:nx, 1:3), Fortran order, as
+ *    interleaved doubles (dnsdata.f90:132, README.md:196-216).
+ *  - one host thread per handle; calls are not thread-safe.  Calls that
+ *    return device-computed scalars synchronise the library's stream.
+ *  - no torch types and no ownership transfer: pointers are read or written
+ *    during the call only.
+ */
+#ifndef CHANNEL_HIP_H
+#define CHANNEL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHAN_ABI_VERSION 1
+
+/* precision modes */
+#define CHAN_FP64 0  /* plain FP64 step: the drop-in */
+#define CHAN_SPLIT 1 /* FP32 transforms/products/stencils, FP64 solves and mean mode */
+
+/* status codes below zero */
+#define CHAN_E_ARG (-1)     /* invalid argument */
+#define CHAN_E_SIZE (-2)    /* unsupported transform size (fftFIT sizes only) */
+#define CHAN_E_NOTFINITE (-3) /* CFL reduction is not finite (NaN/Inf in the field) */
+#define CHAN_E_CPITYPE (-4) /* CPI_type not 0 or 1 (linsolve_blocking.inc:93-95) */
+#define CHAN_E_STATE (-5)   /* call order violated (e.g. step before set_field) */
+#define CHAN_E_NODEV (-6)   /* no HIP device */
+
+/* The run-time part of dns.in (dnsdata.f90:110-122) with ni already
+ * inverted to the viscosity 1/Re (dnsdata.f90:115), plus the precision mode. */
+typedef struct chan_params {
+  int32_t nx, ny, nz;
+  int32_t precision; /* CHAN_FP64 or CHAN_SPLIT */
+  double alfa0, beta0;
+  double ni; /* viscosity */
+  double a, ymin, ymax;
+  int32_t cpi, cpi_type;
+  double gamma;
+  double meanpx, meanpz;
+  double meanflowx, meanflowz;
+  double u0, uN;
+  double deltat, cflmax, time;
+} chan_params;
+
+typedef struct chan_ctx chan_ctx;
+
+/* Scalars of the step (module globals of dnsdata.f90:28-32,68,74,76). */
+typedef struct chan_scalars {
+  double time, deltat;
+  double meanpx, meanpz;
+  double corrpx, corrpz;
+  double fr[3];
+  double cfl; /* running CFL maximum since the last outstats */
+} chan_scalars;
+
+/* Library version and the message of the last failing call. */
+int chan_abi_version(void);
+const char *chan_last_error(void);
+
+/* read_dnsin derived sizes + init_memory + setup_derivatives +
+ * setup_boundary_conditions (dnsdata.f90:110-125,129-158,241-308): builds the
+ * y grid, compact stencils, boundary rows and the factorised D0 band on the
+ * host in FP64, allocates the device state on the current HIP device. */
+int chan_create(const chan_params *p, chan_ctx **out);
+/* free_memory / free_fft (dnsdata.f90:222-229, ffts.f90:110-115) */
+int chan_destroy(chan_ctx *c);
+
+/* sizes[5] = nx, ny, nz, nxd, nzd (nxd, nzd from fftFIT, ffts.f90:78-86) */
+int chan_sizes(const chan_ctx *c, int32_t sizes[5]);
+
+/* Set-up products for inspection: y(-1:ny+1) [ny+3]; der(1:ny-1)%d0,d1,d2,d4
+ * as [(ny-1)][4][5]; boundary stencils d040,d140,d240,d14m1,d24m1,d04n,d14n,
+ * d24n,d14np1,d24np1 as [10][5] (dnsdata.f90:241-276). */
+int chan_get_setup(const chan_ctx *c, double *y, double *der, double *bnd);
+
+/* Host-only variant of the set-up query that needs no device and no
+ * context: sizes[5] as chan_sizes, then y, der, bnd as chan_get_setup, the
+ * eight boundary-condition rows v0bc,v0m1bc,eta0bc,eta0m1bc,vnbc,vnp1bc,
+ * etanbc,etanp1bc as bc[8][5] (dnsdata.f90:290-308) and the UL-factorised
+ * D0 band as d0fac[(ny+1)][5] (dnsdata.f90:277-285).  Any output may be NULL. */
+int chan_setup_host(const chan_params *p, int32_t sizes[5], double *y, double *der, double *bnd, double *bc,
+                    double *d0fac);
+
+/* Field I/O in the reference layout (the payload of read_restart_file /
+ * save_restart_file, dnsdata.f90:677-720, 821-848; the library does no file
+ * I/O).  V has (ny+3)*(2nz+1)*(nx+1)*3 complex128 = 2x doubles. */
+int chan_set_field(chan_ctx *c, const double *V);
+int chan_get_field(chan_ctx *c, double *V);
+
+/* The right-hand sides left by chan_buildrhs, i.e. what the reference holds
+ * in V(1:ny-1, :, :, 1:2) between buildrhs and linsolve (dnsdata.f90:667-671):
+ * R(1:ny-1, -nz:nz, 0:nx, 1:2) complex128, Fortran order. */
+int chan_get_rhs(chan_ctx *c, double *R);
+
+int chan_get_scalars(chan_ctx *c, chan_scalars *s);
+int chan_set_scalars(chan_ctx *c, const chan_scalars *s);
+
+/* channel.f90:95-115: CFL pass over the interior planes (convolutions with
+ * compute_cfl), initial flow rates fr(1:2) and the CPI pressure gradient. */
+int chan_initial(chan_ctx *c);
+
+/* buildrhs(ODE, compute_cfl) (dnsdata.f90:611-673): products of all planes,
+ * RHS stencils and the CN-RK3 time scheme; leaves the right-hand sides of
+ * eta and D2v in the state in place of u, v for linsolve. */
+int chan_buildrhs(chan_ctx *c, const double ode[3], int32_t compute_cfl);
+
+/* linsolve(lambda) (linsolve_blocking.inc:3-107): per-(kx,kz) pentadiagonal
+ * v and eta solves with wall BCs, dv/dy, u and w, and the (0,0) mean-flow
+ * mode with flow-rate correction / CPI. */
+int chan_linsolve(chan_ctx *c, double lambda);
+
+/* outstats (dnsdata.f90:853-881) minus its file I/O: takes the CFL maximum,
+ * sets deltat = cflmax/CFL when cflmax > 0, resets the CFL maximum and fills
+ * the 11 Runtimedata columns (time, dudy bottom, -dudy top, dwdy bottom,
+ * -dwdy top, fr_x, dp/dx, fr_z, dp/dz, CFL, deltat). */
+int chan_outstats(chan_ctx *c, double row[11]);
+
+/* One iteration of the time loop (channel.f90:122-167): wall velocities
+ * u0/uN, three substeps of buildrhs+linsolve with time advance, outstats.
+ * n iterations write n rows. */
+int chan_step(chan_ctx *c, double row[11]);
+int chan_steps(chan_ctx *c, int32_t n, double *rows);
+
+/* Per-kernel timing with HIP events recorded on the library's stream around
+ * every launch (off by default).  chan_kernel_times fills up to cap entries of
+ * names (NUL-terminated, 32 chars each), launch counts and total ms, and
+ * returns the number of kernels in *n. */
+int chan_timing(chan_ctx *c, int32_t enable);
+int chan_kernel_times(chan_ctx *c, int32_t cap, char *names, int64_t *counts, double *total_ms, int32_t *n);
+/* Synchronises the library's stream. */
+int chan_sync(chan_ctx *c);
+
+/* Diagnostic: batched c2c FFT of length n (sign -1 forward, +1 backward,
+ * unnormalised, FFTW conventions) on `batch` contiguous complex128 rows using
+ * the library's device transform. */
+int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -677,13 +677,11 @@ static void all_products(orc_t *o, int compute_cfl, int lo, int hi) {
 /* ----------------------------------------------------------- buildrhs -- */
 /* buildrhs (dnsdata.f90:611-673) with the DD stencil (:609-610), OS/SQ
  * (:476-477) and timescheme (:486).  New rhs goes to V(1..ny-1, :, :, 1:2). */
-void orc_buildrhs(orc_t *o, const double ode[3], int compute_cfl) {
-  int ny = o->ny, nx = o->nx, nz = o->nz, Nz = o->Nz, Nx = o->Nx;
-  all_products(o, compute_cfl, -1, ny + 1);
-  cplx *newrhs = (cplx *)malloc(sizeof(cplx) * (size_t)(ny - 1) * Nx * Nz * 2);
+static void rhs_rows(orc_t *o, const double ode[3], int lo, int hi, cplx *newrhs) {
+  int nx = o->nx, nz = o->nz, Nz = o->Nz, Nx = o->Nx;
   const double ni = o->p.ni, dt = o->deltat;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
-  for (int iy = 1; iy <= ny - 1; iy++) {
+  for (int iy = lo; iy <= hi; iy++) {
     const Di *d = &o->der[iy];
     for (int iz = -nz; iz <= nz; iz++)
       for (int ix = 0; ix <= nx; ix++) {
@@ -763,6 +761,13 @@ void orc_buildrhs(orc_t *o, const double ode[3], int compute_cfl) {
 #undef VV
       }
   }
+}
+
+void orc_buildrhs(orc_t *o, const double ode[3], int compute_cfl) {
+  int ny = o->ny, nx = o->nx, nz = o->nz, Nz = o->Nz, Nx = o->Nx;
+  all_products(o, compute_cfl, -1, ny + 1);
+  cplx *newrhs = (cplx *)malloc(sizeof(cplx) * (size_t)(ny - 1) * Nx * Nz * 2);
+  rhs_rows(o, ode, 1, ny - 1, newrhs);
   /* write-back of the rhs into V(iy,:,:,1:2) (dnsdata.f90:667-671) */
 #pragma omp parallel for num_threads(g_threads) schedule(static)
   for (int iy = 1; iy <= ny - 1; iy++)
@@ -796,9 +801,9 @@ static void applybc_n(double (*EQ)[5], int ny, const double *bcn, const double *
   for (int j = 1; j <= 3; j++) r2[j] = r2[j] - e * bcn[j - 1] / bcn[3];
 }
 
-/* linsolve (linsolve_blocking.inc:3-107) */
-void orc_linsolve(orc_t *o, double lambda) {
-  int ny = o->ny, nx = o->nx, nz = o->nz, Ny = o->Ny;
+/* linsolve (linsolve_blocking.inc:3-107) over the x modes ixlo..ixhi */
+static void linsolve_cols(orc_t *o, double lambda, int ixlo, int ixhi) {
+  int ny = o->ny, nz = o->nz, Ny = o->Ny;
   const double ni = o->p.ni;
 #pragma omp parallel num_threads(g_threads)
   {
@@ -808,7 +813,7 @@ void orc_linsolve(orc_t *o, double lambda) {
     cplx *ucor = (cplx *)malloc(sizeof(cplx) * (size_t)Ny);
     double *fre = (double *)malloc(sizeof(double) * (size_t)Ny);
 #pragma omp for schedule(dynamic, 1)
-    for (int ix = 0; ix <= nx; ix++)
+    for (int ix = ixlo; ix <= ixhi; ix++)
       for (int iz = -nz; iz <= nz; iz++) {
         double al = o->p.alfa0 * ix, be = o->p.beta0 * iz;
         double k2 = (o->p.alfa0 * ix) * (o->p.alfa0 * ix) + (o->p.beta0 * iz) * (o->p.beta0 * iz);
@@ -951,6 +956,38 @@ void orc_linsolve(orc_t *o, double lambda) {
     free(ucor);
     free(fre);
   }
+}
+
+void orc_linsolve(orc_t *o, double lambda) { linsolve_cols(o, lambda, 0, o->nx); }
+
+static double wall_s(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
+/* Wall-clock cost of the three parts of a substep measured on a bounded
+ * sample of the workload: out[0] seconds per plane of products (planes
+ * 1..nplanes), out[1] seconds per row of rhs assembly (rows 3..nplanes-2),
+ * out[2] seconds per x-mode column of the implicit solve (ix = 1..nix).  The
+ * state is left modified: call it on a scratch instance. */
+void orc_time_sample(orc_t *o, int nplanes, int nix, double out[3]) {
+  static const double ode[3] = {120.0 / 32.0, 2.0, 0.0};
+  if (o->deltat == 0) o->deltat = 1e-3;
+  double t0 = wall_s();
+  all_products(o, 0, 1, nplanes);
+  double t1 = wall_s();
+  cplx *newrhs = (cplx *)malloc(sizeof(cplx) * (size_t)(o->ny - 1) * o->Nx * o->Nz * 2);
+  rhs_rows(o, ode, 3, nplanes - 2, newrhs);
+  double t2 = wall_s();
+  free(newrhs);
+  linsolve_cols(o, ode[0] / o->deltat, 1, nix);
+  double t3 = wall_s();
+  out[0] = (t1 - t0) / nplanes;
+  out[1] = (t2 - t1) / (nplanes - 4);
+  out[2] = (t3 - t2) / nix;
 }
 
 /* channel.f90:95-115: pre-loop CFL pass and initial flow rate / CPI */

@@ -62,6 +62,7 @@ void orc_initial(orc_t *o);
 void orc_outstats(orc_t *o, double row[11]);
 void orc_step(orc_t *o, double row[11]);
 void orc_steps(orc_t *o, int n, double *rows);
+void orc_time_sample(orc_t *o, int nplanes, int nix, double out[3]);
 
 /* primitives exposed for pinning tests */
 void orc_fft(int n, int sign, double *inout);

@@ -64,6 +64,7 @@ def lib():
         L.orc_outstats.argtypes = [P, D]
         L.orc_step.argtypes = [P, D]
         L.orc_steps.argtypes = [P, I, D]
+        L.orc_time_sample.argtypes = [P, I, I, D]
         L.orc_fft.argtypes = [I, I, D]
         L.orc_c2r.argtypes = [I, D, D]
         L.orc_r2c.argtypes = [I, D, D]
@@ -192,6 +193,13 @@ class Oracle:
         r = np.zeros((n, 11))
         lib().orc_steps(self.h, int(n), _dp(r))
         return r
+
+    def time_sample(self, nplanes, nix):
+        """Seconds per plane of products, per row of rhs, per x-mode column
+        of the solve, measured on a bounded sample (modifies the state)."""
+        out = np.zeros(3)
+        lib().orc_time_sample(self.h, int(nplanes), int(nix), _dp(out))
+        return out
 
     def yintegr(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
