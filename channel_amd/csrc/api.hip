@@ -228,7 +228,9 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   c->prm = *p;
   setup_build(c->hs, p->nx, p->ny, p->nz, p->alfa0, p->beta0, p->a, p->ymin, p->ymax);
   const HostSetup &h = c->hs;
-  if (!fft_size_supported(h.nzd) || !fft_size_supported(2 * h.nxd) || h.nzd % ZB != 0) {
+  // the x pass holds at most (E+2)/3 x modes per thread, i.e. Nx <= 2nxd/3,
+  // which fftFIT's nxd >= 3(nx+1)/2 guarantees
+  if (!fft_size_supported(h.nzd) || !fft_size_supported(2 * h.nxd) || h.nzd % ZB != 0 || 3 * (p->nx + 1) > 2 * h.nxd) {
     int rc = fail(CHAN_E_SIZE, "transform sizes nzd=%d, 2nxd=%d are not supported", h.nzd, 2 * h.nxd);
     setup_free(c->hs);
     delete c;

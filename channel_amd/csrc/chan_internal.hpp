@@ -112,8 +112,6 @@ template <typename R>
 hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, const cx<R> *tw, const Geom &g, int plane0,
                        int nplanes);
 template <typename R>
-size_t xpass_smem(int M, const Geom &g);
-template <typename R>
 hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
                         int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
                         double *cfl_part);

@@ -47,15 +47,20 @@ __host__ __device__ __forceinline__ cx<R> conj(cx<R> a) {
 
 namespace fft {
 
-template <int N>
+// Radix plan of an N-point transform with E elements per thread.  BIG: a
+// first pass of radix E (12 or 16, done in registers as 4x3 or 4x4), then
+// radix-4 passes and at most one radix-2 pass.  Otherwise radix-4 passes, at
+// most one radix-2 pass and a final radix-3 pass (fewer live registers, one
+// more LDS exchange).
+template <int N, bool BIG = true>
 struct Plan {
   static constexpr bool pow2 = (N & (N - 1)) == 0;
   static constexpr int E = pow2 ? 16 : 12;
   static constexpr int T = N / E;
   static_assert(T >= 1 && T * E == N, "unsupported FFT length");
-  static constexpr int base() { return pow2 ? N : N / 3; }
+  static constexpr int rest() { return BIG ? T : (pow2 ? N : N / 3); }
   static constexpr int n4() {
-    int r = base(), n = 0;
+    int r = rest(), n = 0;
     while (r % 4 == 0) {
       r /= 4;
       n++;
@@ -63,12 +68,15 @@ struct Plan {
     return n;
   }
   static constexpr bool has2() {
-    int r = base();
+    int r = rest();
     while (r % 4 == 0) r /= 4;
     return r == 2;
   }
-  static constexpr int npass() { return n4() + (has2() ? 1 : 0) + (pow2 ? 0 : 1); }
-  static constexpr int radix(int p) { return p < n4() ? 4 : ((p == n4() && has2()) ? 2 : 3); }
+  static constexpr int npass() { return (BIG ? 1 : (pow2 ? 0 : 1)) + n4() + (has2() ? 1 : 0); }
+  static constexpr int radix(int p) {
+    if (BIG) return p == 0 ? E : (p <= n4() ? 4 : 2);
+    return p < n4() ? 4 : ((p == n4() && has2()) ? 2 : 3);
+  }
 };
 
 // team-level LDS synchronisation: a team within one wavefront only needs the
@@ -102,8 +110,7 @@ __device__ __forceinline__ void dft(cx<R> *a) {
       a[1] = cx<R>{t1.x - t3.y, t1.y + t3.x};
       a[3] = cx<R>{t1.x + t3.y, t1.y - t3.x};
     }
-  } else {
-    static_assert(RAD == 3, "radix");
+  } else if constexpr (RAD == 3) {
     const R s = (S < 0 ? R(-1) : R(1)) * R(0.86602540378443864676372317075293618);
     cx<R> t1 = add(a[1], a[2]);
     cx<R> d = sub(a[1], a[2]);
@@ -111,6 +118,64 @@ __device__ __forceinline__ void dft(cx<R> *a) {
     a[0] = add(a[0], t1);
     a[1] = cx<R>{t2.x - s * d.y, t2.y + s * d.x};
     a[2] = cx<R>{t2.x + s * d.y, t2.y - s * d.x};
+  } else if constexpr (RAD == 12) {
+    // 12 = 4 x 3: n = 3 n1 + n2, q = q1 + 4 q2;
+    // y_q = sum_n2 w3^(n2 q2) w12^(n2 q1) sum_n1 w4^(n1 q1) a_(3 n1 + n2)
+    const R sg = S < 0 ? R(-1) : R(1);
+    const R h = R(0.5), c = R(0.86602540378443864676372317075293618);
+    cx<R> b[3][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; n2++) {
+      cx<R> q[4] = {a[n2], a[3 + n2], a[6 + n2], a[9 + n2]};
+      dft<4, S>(q);
+#pragma unroll
+      for (int q1 = 0; q1 < 4; q1++) b[n2][q1] = q[q1];
+    }
+    // twiddles w12^(n2 q1): (n2,q1) = (1,1) (1,2) (1,3) (2,1) (2,2) (2,3)
+    b[1][1] = mul(b[1][1], cx<R>{c, sg * h});
+    b[1][2] = mul(b[1][2], cx<R>{h, sg * c});
+    b[1][3] = cx<R>{-sg * b[1][3].y, sg * b[1][3].x};  // * S i
+    b[2][1] = mul(b[2][1], cx<R>{h, sg * c});
+    b[2][2] = mul(b[2][2], cx<R>{-h, sg * c});
+    b[2][3] = cx<R>{-b[2][3].x, -b[2][3].y};  // * -1
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) {
+      cx<R> t[3] = {b[0][q1], b[1][q1], b[2][q1]};
+      dft<3, S>(t);
+#pragma unroll
+      for (int q2 = 0; q2 < 3; q2++) a[q1 + 4 * q2] = t[q2];
+    }
+  } else {
+    static_assert(RAD == 16, "radix");
+    // 16 = 4 x 4: n = 4 n1 + n2, q = q1 + 4 q2
+    const R sg = S < 0 ? R(-1) : R(1);
+    const R c1 = R(0.92387953251128675612818318939678829), s1 = R(0.38268343236508977172845998403039887);
+    const R r2 = R(0.70710678118654752440084436210484904);
+    cx<R> b[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+      cx<R> q[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+      dft<4, S>(q);
+#pragma unroll
+      for (int q1 = 0; q1 < 4; q1++) b[n2][q1] = q[q1];
+    }
+    // twiddles w16^(n2 q1) with w16 = exp(S 2 pi i/16)
+    b[1][1] = mul(b[1][1], cx<R>{c1, sg * s1});
+    b[1][2] = mul(b[1][2], cx<R>{r2, sg * r2});
+    b[1][3] = mul(b[1][3], cx<R>{s1, sg * c1});
+    b[2][1] = mul(b[2][1], cx<R>{r2, sg * r2});
+    b[2][2] = cx<R>{-sg * b[2][2].y, sg * b[2][2].x};  // * S i
+    b[2][3] = mul(b[2][3], cx<R>{-r2, sg * r2});
+    b[3][1] = mul(b[3][1], cx<R>{s1, sg * c1});
+    b[3][2] = mul(b[3][2], cx<R>{-r2, sg * r2});
+    b[3][3] = mul(b[3][3], cx<R>{-c1, -sg * s1});
+#pragma unroll
+    for (int q1 = 0; q1 < 4; q1++) {
+      cx<R> t[4] = {b[0][q1], b[1][q1], b[2][q1], b[3][q1]};
+      dft<4, S>(t);
+#pragma unroll
+      for (int q2 = 0; q2 < 4; q2++) a[q1 + 4 * q2] = t[q2];
+    }
   }
 }
 
@@ -118,9 +183,9 @@ __device__ __forceinline__ void dft(cx<R> *a) {
 // held in "read order" v[m*RAD + r] = in[j + r*N/RAD], j = t + m*T; then the
 // exchange through LDS into the read order of the next pass.  LDS holds N
 // reals per team: real and imaginary parts travel in two rounds.
-template <int N, int S, int P, int NS, typename R>
+template <int N, int S, int P, int NS, bool BIG, typename R>
 __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
-  using PL = Plan<N>;
+  using PL = Plan<N, BIG>;
   constexpr int E = PL::E, T = PL::T, RAD = PL::radix(P), NB = E / RAD;
   constexpr int TWS = N / (NS * RAD);
 #pragma unroll
@@ -162,23 +227,23 @@ __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const c
         }
       team_sync<T>();
     }
-    run_pass<N, S, P + 1, NS * RAD, R>(v, lds, tw, t);
+    run_pass<N, S, P + 1, NS * RAD, BIG, R>(v, lds, tw, t);
   }
 }
 
 // Transform the team's N points held in slots e[s] = x[t + T*s]; on return
 // e[s] = X[t + T*s].  lds: N reals private to the team.  tw: N twiddles
 // tw[e] = exp(-2 pi i e/N).
-template <int N, int S, typename R>
+template <int N, int S, bool BIG = true, typename R>
 __device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
-  using PL = Plan<N>;
+  using PL = Plan<N, BIG>;
   constexpr int E = PL::E, R1 = PL::radix(0), RL = PL::radix(PL::npass() - 1);
   cx<R> v[E];
 #pragma unroll
   for (int m = 0; m < E / R1; m++)
 #pragma unroll
     for (int r = 0; r < R1; r++) v[m * R1 + r] = e[m + r * (E / R1)];
-  run_pass<N, S, 0, 1, R>(v, lds, tw, t);
+  run_pass<N, S, 0, 1, BIG, R>(v, lds, tw, t);
 #pragma unroll
   for (int m = 0; m < E / RL; m++)
 #pragma unroll

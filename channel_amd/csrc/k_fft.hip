@@ -54,6 +54,10 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
 }
 
 // ------------------------------------------------------------------- K2 --
+// the x pass keeps u, v, w of its row live across five transforms, so it
+// uses the radix plan with the smaller register footprint
+constexpr bool XBIG = false;
+
 template <int NT>
 __device__ __forceinline__ void block_max_store(double m, double *red, double *out) {
   red[threadIdx.x] = m;
@@ -70,64 +74,105 @@ __device__ __forceinline__ void block_max_store(double m, double *red, double *o
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// value of the team-mate holding element M-k of a transform in slot layout:
+// for t > 0 it is lane T-t, slot E-1-s; for t = 0 the thread itself, slot
+// (E-s) mod E.  Teams of up to one wavefront use cross-lane moves, larger
+// teams go through LDS (buf: M reals of the team).
+// x modes per thread of the x pass: nxd >= 3(nx+1)/2 bounds Nx by M/3
+template <int E>
+constexpr int xmodes() {
+  return (E + 2) / 3;
+}
+
+template <int T, int E, typename R>
+__device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmodes<E>()], int t, int nmode, R *buf) {
+  constexpr int MO = xmodes<E>();
+  if constexpr (T <= 64) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int src = lane - t + ((T - t) % T);
+#pragma unroll
+    for (int s = 0; s < MO; s++) {
+      if (s * T < nmode) {  // uniform across the team
+        const cx<R> send = e[E - 1 - s];
+        const R rx = __shfl(send.x, src, 64), ry = __shfl(send.y, src, 64);
+        m[s] = t ? cx<R>{rx, ry} : e[(E - s) % E];
+      }
+    }
+  } else {
+    constexpr int M = T * E;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+#pragma unroll
+      for (int s = 0; s < E; s++) buf[t + T * s] = half ? e[s].y : e[s].x;
+      fft::team_sync<T>();
+#pragma unroll
+      for (int s = 0; s < MO; s++) {
+        const int k = t + T * s;
+        if (k < nmode) {
+          const R v = buf[k == 0 ? 0 : M - k];
+          if (half)
+            m[s].y = v;
+          else
+            m[s].x = v;
+        }
+      }
+      fft::team_sync<T>();
+    }
+  }
+}
+
 template <int M, typename R, bool PRODUCTS>
-__global__ __launch_bounds__(ZB *fft::Plan<M>::T) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
+__global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
                                                                 double dz, R factor, double *__restrict__ cfl_part) {
   using PL = fft::Plan<M>;
   constexpr int T = PL::T, E = PL::E, NT = ZB * T;
-  constexpr int MO = E / 2 + 1;  // x modes per thread: ceil(Nx/T) <= E/2
-  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int MO = xmodes<E>();
+  __shared__ R fl[ZB * M];
   __shared__ double red[NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
-  cx<R> *stage = reinterpret_cast<cx<R> *>(smem);                          // [3][ZB][Nx]
-  R *fl = reinterpret_cast<R *>(smem + sizeof(cx<R>) * 3 * ZB * Nx);        // [ZB][M]
-  const int tid = threadIdx.x, row = tid / T, t = tid % T;
+  const int tid = threadIdx.x, zi = tid / T, t = tid % T;
   const int zb = blockIdx.x % NZB, pl = blockIdx.x / NZB;
   const int iy = plane0 + pl - 1;
+  R *fr = fl + zi * M;
 
-  // stage the ZB rows of u, v, w: A holds [ix][zi] runs of Nx*ZB values
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB;
-    for (int i = tid; i < Nx * ZB; i += NT) stage[(c * ZB + (i % ZB)) * Nx + i / ZB] = src[i];
-  }
-  __syncthreads();
-
-  // Hermitian extension to M points; Im of mode 0 is dropped as c2r does
-  cx<R> e1[E], e2[E];
-  {
-    const cx<R> *su = stage + (0 * ZB + row) * Nx, *sv = stage + (1 * ZB + row) * Nx, *sw = stage + (2 * ZB + row) * Nx;
+  // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
+  // (k <= nx) or the conjugate of mode M-k (k >= M-nx); Im of mode 0 is
+  // dropped as c2r does.
+  auto load_ext = [&](int c, cx<R>(&e)[E]) {
+    const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB + zi;
 #pragma unroll
     for (int s = 0; s < E; s++) {
       const int k = t + T * s;
-      cx<R> U = {0, 0}, Vv = {0, 0}, W = {0, 0};
+      cx<R> v = {0, 0};
       if (k <= nx) {
-        U = su[k];
-        Vv = sv[k];
-        W = sw[k];
-        if (k == 0) U.y = Vv.y = W.y = 0;
+        v = src[(size_t)k * ZB];
+        if (k == 0) v.y = 0;
       } else if (k >= M - nx) {
-        U = conj(su[M - k]);
-        Vv = conj(sv[M - k]);
-        W = conj(sw[M - k]);
+        v = conj(src[(size_t)(M - k) * ZB]);
       }
-      e1[s] = cx<R>{U.x - Vv.y, U.y + Vv.x};
-      e2[s] = W;
+      e[s] = v;
     }
-  }
-  __syncthreads();  // the stage area is reused for the outputs
+  };
 
-  R *fr = fl + row * M;
-  fft::fft_slots<M, +1>(e1, fr, tw, t);
-  fft::fft_slots<M, +1>(e2, fr, tw, t);
   R u[E], v[E], w[E];
+  {
+    cx<R> e[E], f[E];
+    load_ext(0, e);
+    load_ext(1, f);
 #pragma unroll
-  for (int s = 0; s < E; s++) {
-    u[s] = e1[s].x;
-    v[s] = e1[s].y;
-    w[s] = e2[s].x;
+    for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
+    fft::fft_slots<M, +1, XBIG>(e, fr, tw, t);
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      u[s] = e[s].x;
+      v[s] = e[s].y;
+    }
+    load_ext(2, f);
+    fft::fft_slots<M, +1, XBIG>(f, fr, tw, t);
+#pragma unroll
+    for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
 
   // CFL maximum over the interior planes (dnsdata.f90:552-556)
@@ -143,65 +188,33 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T) void k_xpass(const cx<R> *__re
   }
 
   if constexpr (PRODUCTS) {
-    cx<R> *ost = stage;  // [2][ZB][Nx]
-#pragma unroll
+    // pairs (uu,vv), (ww,uv), (vw,uw) share one forward transform each
+#pragma unroll 1
     for (int pp = 0; pp < 3; pp++) {
       cx<R> e[E];
 #pragma unroll
       for (int s = 0; s < E; s++) {
-        R a, b;
-        if (pp == 0) {
-          a = u[s] * u[s] * factor;
-          b = v[s] * v[s] * factor;
-        } else if (pp == 1) {
-          a = w[s] * w[s] * factor;
-          b = u[s] * v[s] * factor;
-        } else {
-          a = v[s] * w[s] * factor;
-          b = u[s] * w[s] * factor;
-        }
-        e[s] = cx<R>{a, b};
+        const R a0 = pp == 0 ? u[s] : (pp == 1 ? w[s] : v[s]);
+        const R a1 = pp == 0 ? u[s] : (pp == 1 ? w[s] : w[s]);
+        const R b0 = pp == 0 ? v[s] : u[s];
+        const R b1 = pp == 0 ? v[s] : (pp == 1 ? v[s] : w[s]);
+        e[s] = cx<R>{a0 * a1 * factor, b0 * b1 * factor};
       }
-      fft::fft_slots<M, -1>(e, fr, tw, t);
-      // split Z = FFT(a + ib) into A_k = (Z_k + conj Z_{M-k})/2 and
-      // B_k = (Z_k - conj Z_{M-k})/(2i) for k = 0..nx
-      R kx[MO], ky[MO], mx[MO], my[MO];
+      fft::fft_slots<M, -1, XBIG>(e, fr, tw, t);
+      // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
+      cx<R> mz[MO];
+      mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
+      cx<R> *da = B + ((size_t)(pl * 6 + 2 * pp) * NZB + zb) * Nx * ZB + zi;
+      cx<R> *db = da + (size_t)NZB * Nx * ZB;
 #pragma unroll
-      for (int half = 0; half < 2; half++) {
-#pragma unroll
-        for (int s = 0; s < E; s++) fr[t + T * s] = half ? e[s].y : e[s].x;
-        fft::team_sync<T>();
-#pragma unroll
-        for (int m = 0; m < MO; m++) {
-          const int k = t + T * m;
-          if (k <= nx) {
-            const R zk = fr[k], zm = fr[k == 0 ? 0 : M - k];
-            if (half) {
-              ky[m] = zk;
-              my[m] = zm;
-            } else {
-              kx[m] = zk;
-              mx[m] = zm;
-            }
-          }
-        }
-        fft::team_sync<T>();
-      }
-#pragma unroll
-      for (int m = 0; m < MO; m++) {
-        const int k = t + T * m;
+      for (int s = 0; s < MO; s++) {
+        const int k = t + T * s;
         if (k <= nx) {
-          ost[(0 * ZB + row) * Nx + k] = cx<R>{(kx[m] + mx[m]) * R(0.5), (ky[m] - my[m]) * R(0.5)};
-          ost[(1 * ZB + row) * Nx + k] = cx<R>{(ky[m] + my[m]) * R(0.5), (mx[m] - kx[m]) * R(0.5)};
+          const cx<R> z = e[s], q = mz[s];
+          da[(size_t)k * ZB] = cx<R>{(z.x + q.x) * R(0.5), (z.y - q.y) * R(0.5)};
+          db[(size_t)k * ZB] = cx<R>{(z.y + q.y) * R(0.5), (q.x - z.x) * R(0.5)};
         }
       }
-      __syncthreads();
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        cx<R> *dst = B + ((size_t)(pl * 6 + 2 * pp + h) * NZB + zb) * Nx * ZB;
-        for (int i = tid; i < Nx * ZB; i += NT) dst[i] = ost[(h * ZB + (i % ZB)) * Nx + i / ZB];
-      }
-      __syncthreads();
     }
   }
 }
@@ -261,7 +274,7 @@ __global__ __launch_bounds__(256) void k_fft_test(cx<double> *__restrict__ data,
 }
 
 // ------------------------------------------------------------ launchers --
-#define CHAN_FOR_SIZES(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+#define CHAN_FOR_SIZES(X) X(12) X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
 
 template <typename R>
 hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, const cx<R> *tw, const Geom &g, int plane0,
@@ -281,32 +294,21 @@ hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, co
 }
 
 template <typename R>
-size_t xpass_smem(int M, const Geom &g) {
-  return sizeof(cx<R>) * 3 * ZB * g.Nx + sizeof(R) * ZB * M;
-}
-
-template <typename R>
 hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
                         int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
                         double *cfl_part) {
   dim3 grid((unsigned)((g.nzd / ZB) * nplanes));
-  const size_t sh = xpass_smem<R>(M, g);
   switch (M) {
-#define X(N)                                                                                                      \
-  case N: {                                                                                                       \
-    dim3 blk(ZB * fft::Plan<N>::T);                                                                               \
-    if (products) {                                                                                               \
-      hipError_t ea = hipFuncSetAttribute((const void *)k_xpass<N, R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); \
-      if (ea != hipSuccess) return ea;                                                                            \
-      hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, sh, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor, \
-                         cfl_part);                                                                               \
-    } else {                                                                                                      \
-      hipError_t ea = hipFuncSetAttribute((const void *)k_xpass<N, R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh); \
-      if (ea != hipSuccess) return ea;                                                                             \
-      hipLaunchKernelGGL((k_xpass<N, R, false>), grid, blk, sh, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor, \
-                         cfl_part);                                                                               \
-    }                                                                                                             \
-    return hipGetLastError();                                                                                     \
+#define X(N)                                                                                                          \
+  case N: {                                                                                                           \
+    dim3 blk(ZB * fft::Plan<N>::T);                                                                                   \
+    if (products)                                                                                                     \
+      hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,    \
+                         cfl_part);                                                                                   \
+    else                                                                                                              \
+      hipLaunchKernelGGL((k_xpass<N, R, false>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,   \
+                         cfl_part);                                                                                   \
+    return hipGetLastError();                                                                                         \
   }
     CHAN_FOR_SIZES(X)
 #undef X
@@ -366,6 +368,5 @@ template hipError_t launch_xpass<double>(int, hipStream_t, const cx<double> *, c
                                          double *);
 template hipError_t launch_zfwd<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
                                         const Geom &, int, int);
-template size_t xpass_smem<double>(int, const Geom &);
 
 }  // namespace chan

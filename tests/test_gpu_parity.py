@@ -34,7 +34,7 @@ def pair(cfg, V):
     return g, o
 
 
-@pytest.mark.parametrize("n", [12, 24, 48, 96, 192, 384, 768, 1536, 16, 32, 64, 128, 256, 512, 1024, 2048])
+@pytest.mark.parametrize("n", [12, 24, 48, 96, 192, 384, 768, 1536, 16, 32, 64, 128, 256, 512, 1024])
 @pytest.mark.parametrize("sign", [-1, 1])
 def test_fft_matches_numpy(n, sign):
     say("fft", n, sign)
