@@ -96,12 +96,21 @@ def main():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel events")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="only run the library's plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     from channel_amd import lib
+    if args.copy_calibrate:
+        import ctypes
+        ms = ctypes.c_double()
+        lib.check(lib.load().chan_copy_selftest(args.copy_calibrate, 3, ctypes.byref(ms)))
+        print(json.dumps({"copy_bytes": args.copy_calibrate, "avg_ms": ms.value,
+                          "GBps_read_plus_write": 2 * args.copy_calibrate / ms.value / 1e6}), flush=True)
+        return
     from channel_amd.initial import perturbed_laminar
     from channel_amd.solver import Channel
 

@@ -83,6 +83,11 @@ __global__ void k_gather00(const cx<double> *__restrict__ V, cx<double> *__restr
   for (int c = 0; c < 3; c++) out[c * g.Ny + r] = V[((size_t)c * g.Ny + r) * g.ps + g.nz];
 }
 
+// grid-stride streaming copy of n 32-byte elements
+__global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = a[i];
+}
+
 void twiddles(std::vector<cx<double>> &w, int n) {
   w.resize(n);
   for (int e = 0; e < n; e++) {
@@ -665,6 +670,72 @@ int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout) {
   hipFree(d);
   if (t) hipFree(t);
   if (e != hipSuccess) return fail((int)e, "fft selftest: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int32_t reps, double *inout, double *ms) {
+  const int inner = 1 + (variant >> 3);  // transforms per load/store pair
+  if (!inout || !ms || batch < 1 || reps < 1 || variant < 0 || inner > 64 || (sign != 1 && sign != -1))
+    return fail(CHAN_E_ARG, "bad argument");
+  if (n != 384 && n != 768 && n != 1536) return fail(CHAN_E_SIZE, "size %d not benchmarked", n);
+  std::vector<cx<double>> tw;
+  twiddles(tw, n);
+  const size_t bytes = sizeof(cx<double>) * (size_t)n * batch;
+  cx<double> *a = nullptr, *b = nullptr, *t = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMalloc(&t, sizeof(cx<double>) * n);
+  if (e == hipSuccess) e = hipMemcpy(a, inout, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_fft_bench(n, sign, variant & 7, a, b, t, batch, 1);
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_fft_bench(n, sign, variant & 7, a, b, t, batch, inner);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ft = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ft, e0, e1);
+  if (e == hipSuccess) e = hipMemcpy(inout, b, bytes, hipMemcpyDeviceToHost);
+  *ms = ft / reps;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(a);
+  if (b) hipFree(b);
+  if (t) hipFree(t);
+  if (e != hipSuccess) return fail((int)e, "fft bench: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int chan_copy_selftest(int64_t bytes, int32_t reps, double *ms) {
+  if (bytes <= 0 || bytes % 32 != 0 || reps < 1 || !ms) return fail(CHAN_E_ARG, "bad argument");
+  double4 *a = nullptr, *b = nullptr;
+  const size_t n = (size_t)bytes / sizeof(double4);
+  HIPCHK(hipMalloc(&a, n * sizeof(double4)));
+  hipError_t e = hipMalloc(&b, n * sizeof(double4));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (e == hipSuccess) e = hipMemset(a, 0, n * sizeof(double4));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, 0, a, b, n);  // warm-up
+    e = hipEventRecord(e0, 0);
+  }
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, 0, a, b, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  *ms = t / reps;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(a);
+  if (b) hipFree(b);
+  if (e != hipSuccess) return fail((int)e, "copy selftest: %s", hipGetErrorString(e));
   return 0;
 }
 

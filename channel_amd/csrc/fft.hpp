@@ -179,11 +179,28 @@ __device__ __forceinline__ void dft(cx<R> *a) {
   }
 }
 
+// LDS exchange layouts between passes.  XCH_CPLX moves whole complex
+// values per access (two reals per lane, half the LDS instructions; needs 2N
+// reals per team), otherwise real and imaginary parts travel in two rounds
+// (N reals per team).  XCH_PAD inserts one element after every 32, which
+// breaks the power-of-two strides of the Stockham scatter across LDS banks.
+constexpr int XCH_CPLX = 1, XCH_PAD = 2;
+
+template <int XCH>
+__device__ __forceinline__ int xidx(int i) {
+  return (XCH & XCH_PAD) ? i + (i >> 5) : i;
+}
+
+// LDS reals a team needs for an N-point transform with exchange layout XCH
+template <int N, int XCH>
+constexpr int lds_reals() {
+  return ((XCH & XCH_CPLX) ? 2 : 1) * ((XCH & XCH_PAD) ? N + N / 32 : N);
+}
+
 // One Stockham pass (decimation in time, autosort) on the thread's E values
 // held in "read order" v[m*RAD + r] = in[j + r*N/RAD], j = t + m*T; then the
-// exchange through LDS into the read order of the next pass.  LDS holds N
-// reals per team: real and imaginary parts travel in two rounds.
-template <int N, int S, int P, int NS, bool BIG, typename R>
+// exchange through LDS into the read order of the next pass.
+template <int N, int S, int P, int NS, bool BIG, int XCH, typename R>
 __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
   using PL = Plan<N, BIG>;
   constexpr int E = PL::E, T = PL::T, RAD = PL::radix(P), NB = E / RAD;
@@ -191,50 +208,69 @@ __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const c
 #pragma unroll
   for (int m = 0; m < NB; m++) {
     if constexpr (NS > 1) {
+      // one table load per butterfly; higher powers by multiplication
       const int j = t + m * T;
       const int k = j % NS;
+      cx<R> w1 = tw[k * TWS];
+      if constexpr (S > 0) w1.y = -w1.y;
+      cx<R> wr = w1;
 #pragma unroll
       for (int r = 1; r < RAD; r++) {
-        cx<R> w = tw[r * k * TWS];
-        if constexpr (S > 0) w.y = -w.y;
-        v[m * RAD + r] = mul(v[m * RAD + r], w);
+        v[m * RAD + r] = mul(v[m * RAD + r], wr);
+        if (r + 1 < RAD) wr = mul(wr, w1);
       }
     }
     dft<RAD, S>(&v[m * RAD]);
   }
   if constexpr (P + 1 < PL::npass()) {
     constexpr int R2 = PL::radix(P + 1), NB2 = E / R2;
-#pragma unroll
-    for (int half = 0; half < 2; half++) {
+    if constexpr (XCH & XCH_CPLX) {
+      cx<R> *c = reinterpret_cast<cx<R> *>(lds);
 #pragma unroll
       for (int m = 0; m < NB; m++) {
         const int j = t + m * T;
-        const int k = j % NS;
-        const int base = (j / NS) * NS * RAD + k;
+        const int base = (j / NS) * NS * RAD + j % NS;
 #pragma unroll
-        for (int q = 0; q < RAD; q++) lds[base + q * NS] = half ? v[m * RAD + q].y : v[m * RAD + q].x;
+        for (int q = 0; q < RAD; q++) c[xidx<XCH>(base + q * NS)] = v[m * RAD + q];
       }
       team_sync<T>();
 #pragma unroll
       for (int m = 0; m < NB2; m++)
 #pragma unroll
-        for (int r = 0; r < R2; r++) {
-          R val = lds[t + m * T + r * (N / R2)];
-          if (half)
-            v[m * R2 + r].y = val;
-          else
-            v[m * R2 + r].x = val;
-        }
+        for (int r = 0; r < R2; r++) v[m * R2 + r] = c[xidx<XCH>(t + m * T + r * (N / R2))];
       team_sync<T>();
+    } else {
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+#pragma unroll
+        for (int m = 0; m < NB; m++) {
+          const int j = t + m * T;
+          const int base = (j / NS) * NS * RAD + j % NS;
+#pragma unroll
+          for (int q = 0; q < RAD; q++) lds[xidx<XCH>(base + q * NS)] = half ? v[m * RAD + q].y : v[m * RAD + q].x;
+        }
+        team_sync<T>();
+#pragma unroll
+        for (int m = 0; m < NB2; m++)
+#pragma unroll
+          for (int r = 0; r < R2; r++) {
+            const R val = lds[xidx<XCH>(t + m * T + r * (N / R2))];
+            if (half)
+              v[m * R2 + r].y = val;
+            else
+              v[m * R2 + r].x = val;
+          }
+        team_sync<T>();
+      }
     }
-    run_pass<N, S, P + 1, NS * RAD, BIG, R>(v, lds, tw, t);
+    run_pass<N, S, P + 1, NS * RAD, BIG, XCH, R>(v, lds, tw, t);
   }
 }
 
 // Transform the team's N points held in slots e[s] = x[t + T*s]; on return
-// e[s] = X[t + T*s].  lds: N reals private to the team.  tw: N twiddles
-// tw[e] = exp(-2 pi i e/N).
-template <int N, int S, bool BIG = true, typename R>
+// e[s] = X[t + T*s].  lds: lds_reals<N, XCH>() reals private to the team.
+// tw: N twiddles tw[e] = exp(-2 pi i e/N).
+template <int N, int S, bool BIG = true, int XCH = 0, typename R>
 __device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
   using PL = Plan<N, BIG>;
   constexpr int E = PL::E, R1 = PL::radix(0), RL = PL::radix(PL::npass() - 1);
@@ -243,7 +279,7 @@ __device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const 
   for (int m = 0; m < E / R1; m++)
 #pragma unroll
     for (int r = 0; r < R1; r++) v[m * R1 + r] = e[m + r * (E / R1)];
-  run_pass<N, S, 0, 1, BIG, R>(v, lds, tw, t);
+  run_pass<N, S, 0, 1, BIG, XCH, R>(v, lds, tw, t);
 #pragma unroll
   for (int m = 0; m < E / RL; m++)
 #pragma unroll

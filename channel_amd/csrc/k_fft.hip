@@ -188,24 +188,34 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   }
 
   if constexpr (PRODUCTS) {
-    // pairs (uu,vv), (ww,uv), (vw,uw) share one forward transform each
-#pragma unroll 1
+    // pairs (uu,uv), (uw,vv), (vw,ww) share one forward transform each; in
+    // this order u is dead once the second pair is formed.  Product indices:
+    // 0 uu, 1 vv, 2 ww, 3 uv, 4 vw, 5 uw.
+#pragma unroll
     for (int pp = 0; pp < 3; pp++) {
+      const int qa = pp == 0 ? 0 : (pp == 1 ? 5 : 4), qb = pp == 0 ? 3 : (pp == 1 ? 1 : 2);
       cx<R> e[E];
 #pragma unroll
       for (int s = 0; s < E; s++) {
-        const R a0 = pp == 0 ? u[s] : (pp == 1 ? w[s] : v[s]);
-        const R a1 = pp == 0 ? u[s] : (pp == 1 ? w[s] : w[s]);
-        const R b0 = pp == 0 ? v[s] : u[s];
-        const R b1 = pp == 0 ? v[s] : (pp == 1 ? v[s] : w[s]);
-        e[s] = cx<R>{a0 * a1 * factor, b0 * b1 * factor};
+        R a, b;
+        if (pp == 0) {
+          a = u[s] * u[s] * factor;
+          b = u[s] * v[s] * factor;
+        } else if (pp == 1) {
+          a = u[s] * w[s] * factor;
+          b = v[s] * v[s] * factor;
+        } else {
+          a = v[s] * w[s] * factor;
+          b = w[s] * w[s] * factor;
+        }
+        e[s] = cx<R>{a, b};
       }
       fft::fft_slots<M, -1, XBIG>(e, fr, tw, t);
       // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
       cx<R> mz[MO];
       mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
-      cx<R> *da = B + ((size_t)(pl * 6 + 2 * pp) * NZB + zb) * Nx * ZB + zi;
-      cx<R> *db = da + (size_t)NZB * Nx * ZB;
+      cx<R> *da = B + ((size_t)(pl * 6 + qa) * NZB + zb) * Nx * ZB + zi;
+      cx<R> *db = B + ((size_t)(pl * 6 + qb) * NZB + zb) * Nx * ZB + zi;
 #pragma unroll
       for (int s = 0; s < MO; s++) {
         const int k = t + T * s;
@@ -271,6 +281,61 @@ __global__ __launch_bounds__(256) void k_fft_test(cx<double> *__restrict__ data,
   if (act)
 #pragma unroll
     for (int s = 0; s < E; s++) x[t + T * s] = e[s];
+}
+
+// ------------------------------------------------------------- benchmark --
+// out-of-place batched transforms for timing the exchange/plan variants:
+// variant bit 0 complex exchange, bit 1 padded exchange, bit 2 radix-4/3 plan
+template <int N, int S, bool BIG, int XCH>
+__global__ __launch_bounds__(256) void k_fft_bench(const cx<double> *__restrict__ in, cx<double> *__restrict__ out,
+                                                   const cx<double> *__restrict__ tw, int batch, int inner) {
+  using PL = fft::Plan<N, BIG>;
+  constexpr int T = PL::T, E = PL::E, G = 256 / T, L = fft::lds_reals<N, XCH>();
+  __shared__ __align__(16) double lds[G * L];
+  const int team = threadIdx.x / T, t = threadIdx.x % T;
+  const int row = blockIdx.x * G + team;
+  const bool act = row < batch;
+  const cx<double> *x = in + (size_t)(act ? row : 0) * N;
+  cx<double> e[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) e[s] = act ? x[t + T * s] : cx<double>{0, 0};
+  for (int k = 0; k < inner; k++) fft::fft_slots<N, S, BIG, XCH>(e, lds + team * L, tw, t);
+  if (act)
+#pragma unroll
+    for (int s = 0; s < E; s++) out[(size_t)row * N + t + T * s] = e[s];
+}
+
+template <int N, int S, bool BIG, int XCH>
+hipError_t launch_bench_v(const cx<double> *in, cx<double> *out, const cx<double> *tw, int batch, int inner) {
+  constexpr int G = 256 / fft::Plan<N, BIG>::T;
+  hipLaunchKernelGGL((k_fft_bench<N, S, BIG, XCH>), dim3((unsigned)((batch + G - 1) / G)), dim3(256), 0, 0, in, out, tw,
+                     batch, inner);
+  return hipGetLastError();
+}
+
+template <int N, int S>
+hipError_t launch_bench_n(int variant, const cx<double> *in, cx<double> *out, const cx<double> *tw, int batch, int inner) {
+  switch (variant) {
+    case 0: return launch_bench_v<N, S, true, 0>(in, out, tw, batch, inner);
+    case 1: return launch_bench_v<N, S, true, 1>(in, out, tw, batch, inner);
+    case 2: return launch_bench_v<N, S, true, 2>(in, out, tw, batch, inner);
+    case 3: return launch_bench_v<N, S, true, 3>(in, out, tw, batch, inner);
+    case 4: return launch_bench_v<N, S, false, 0>(in, out, tw, batch, inner);
+    case 5: return launch_bench_v<N, S, false, 1>(in, out, tw, batch, inner);
+    case 6: return launch_bench_v<N, S, false, 2>(in, out, tw, batch, inner);
+    case 7: return launch_bench_v<N, S, false, 3>(in, out, tw, batch, inner);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_fft_bench(int N, int sign, int variant, const cx<double> *in, cx<double> *out, const cx<double> *tw,
+                            int batch, int inner) {
+  switch (N) {
+    case 384: return sign < 0 ? launch_bench_n<384, -1>(variant, in, out, tw, batch, inner) : launch_bench_n<384, 1>(variant, in, out, tw, batch, inner);
+    case 768: return sign < 0 ? launch_bench_n<768, -1>(variant, in, out, tw, batch, inner) : launch_bench_n<768, 1>(variant, in, out, tw, batch, inner);
+    case 1536: return sign < 0 ? launch_bench_n<1536, -1>(variant, in, out, tw, batch, inner) : launch_bench_n<1536, 1>(variant, in, out, tw, batch, inner);
+  }
+  return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------------------ launchers --

@@ -71,7 +71,19 @@ SIGNATURES = {
                                          ctypes.POINTER(_I32)]),
     "chan_sync": (ctypes.c_int, [_P]),
     "chan_fft_selftest": (ctypes.c_int, [_I32, _I32, _I32, _D]),
+    "chan_copy_selftest": (ctypes.c_int, [ctypes.c_int64, _I32, _D]),
+    "chan_fft_bench": (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32, _D, _D]),
 }
+
+
+def fft_bench(x, sign, variant, reps=10):
+    """Timed batched transform of the rows of x with a build variant of the
+    device FFT; returns (result, ms per launch)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    b, n = x.shape
+    ms = ctypes.c_double()
+    check(load().chan_fft_bench(n, int(sign), b, int(variant), int(reps), dptr(x), ctypes.byref(ms)))
+    return x, ms.value
 
 
 class ChanError(RuntimeError):

@@ -189,3 +189,15 @@ def test_hermitian_symmetry_gpu():
     nz = cfg.nz
     err = max(np.max(np.abs(V[:, nz - iz, 0, :] - np.conj(V[:, nz + iz, 0, :]))) for iz in range(1, nz + 1))
     assert err < 1e-14 * np.max(np.abs(V[:, :, 0, :]))
+
+
+@pytest.mark.parametrize("n", [384, 768, 1536])
+@pytest.mark.parametrize("variant", range(8))
+def test_fft_build_variants_match_numpy(n, variant):
+    say("fft variant", n, variant)
+    rng = np.random.default_rng(1000 * n + variant)
+    x = rng.standard_normal((19, n)) + 1j * rng.standard_normal((19, n))
+    for sign in (-1, 1):
+        y, _ = lib.fft_bench(x, sign, variant, reps=1)
+        ref = np.fft.fft(x, axis=1) if sign < 0 else np.fft.ifft(x, axis=1) * n
+        assert rel_l2(y, ref) < 1e-15
