@@ -18,12 +18,16 @@
 namespace chan {
 
 // ------------------------------------------------------------------- K1 --
+// the z transforms are bound by HBM traffic; the padded split exchange keeps
+// their LDS footprint small
+constexpr int ZXCH = fft::XCH_PAD;
+
 template <int NZ, typename R>
 __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, cx<R> *__restrict__ A,
                                               const cx<R> *__restrict__ tw, Geom g, int plane0) {
   using PL = fft::Plan<NZ>;
   constexpr int T = PL::T, E = PL::E, G = 256 / T;
-  __shared__ R lds[G * NZ];
+  __shared__ __align__(16) R lds[G * fft::lds_reals<NZ, ZXCH>()];
   const int team = threadIdx.x / T, t = threadIdx.x % T;
   const int nxg = (g.Nx + G - 1) / G;
   int b = blockIdx.x;
@@ -42,7 +46,7 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
     if (act && iz >= 0) v = col[iz];
     e[s] = cx<R>{(R)v.x, (R)v.y};
   }
-  fft::fft_slots<NZ, +1>(e, lds + team * NZ, tw, t);
+  fft::fft_slots<NZ, +1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
   if (act) {
     cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
 #pragma unroll
@@ -54,9 +58,11 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
 }
 
 // ------------------------------------------------------------------- K2 --
-// the x pass keeps u, v, w of its row live across five transforms, so it
-// uses the radix plan with the smaller register footprint
-constexpr bool XBIG = false;
+// the x pass runs five transforms per row and is bound by their arithmetic
+// and LDS exchanges: radix-12 first pass, whole-complex padded exchanges
+// (measured fastest; 2 waves/SIMD either way, u, v, w stay live)
+constexpr bool XBIG = true;
+constexpr int XXCH = fft::XCH_CPLX | fft::XCH_PAD;
 
 template <int NT>
 __device__ __forceinline__ void block_max_store(double m, double *red, double *out) {
@@ -129,13 +135,13 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   using PL = fft::Plan<M>;
   constexpr int T = PL::T, E = PL::E, NT = ZB * T;
   constexpr int MO = xmodes<E>();
-  __shared__ R fl[ZB * M];
+  __shared__ __align__(16) R fl[ZB * fft::lds_reals<M, XXCH>()];
   __shared__ double red[NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
   const int tid = threadIdx.x, zi = tid / T, t = tid % T;
   const int zb = blockIdx.x % NZB, pl = blockIdx.x / NZB;
   const int iy = plane0 + pl - 1;
-  R *fr = fl + zi * M;
+  R *fr = fl + zi * fft::lds_reals<M, XXCH>();
 
   // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
   // (k <= nx) or the conjugate of mode M-k (k >= M-nx); Im of mode 0 is
@@ -163,14 +169,14 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
     load_ext(1, f);
 #pragma unroll
     for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
-    fft::fft_slots<M, +1, XBIG>(e, fr, tw, t);
+    fft::fft_slots<M, +1, XBIG, XXCH>(e, fr, tw, t);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       u[s] = e[s].x;
       v[s] = e[s].y;
     }
     load_ext(2, f);
-    fft::fft_slots<M, +1, XBIG>(f, fr, tw, t);
+    fft::fft_slots<M, +1, XBIG, XXCH>(f, fr, tw, t);
 #pragma unroll
     for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
@@ -210,7 +216,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
         }
         e[s] = cx<R>{a, b};
       }
-      fft::fft_slots<M, -1, XBIG>(e, fr, tw, t);
+      fft::fft_slots<M, -1, XBIG, XXCH>(e, fr, tw, t);
       // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
       cx<R> mz[MO];
       mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
@@ -235,7 +241,7 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
                                               const cx<R> *__restrict__ tw, Geom g, int plane0) {
   using PL = fft::Plan<NZ>;
   constexpr int T = PL::T, E = PL::E, G = 256 / T;
-  __shared__ R lds[G * NZ];
+  __shared__ __align__(16) R lds[G * fft::lds_reals<NZ, ZXCH>()];
   const int team = threadIdx.x / T, t = threadIdx.x % T;
   const int nxg = (g.Nx + G - 1) / G;
   int b = blockIdx.x;
@@ -251,7 +257,7 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
     const int z = t + T * s;
     e[s] = act ? src[((size_t)(z / ZB) * g.Nx + ix) * ZB + (z % ZB)] : cx<R>{0, 0};
   }
-  fft::fft_slots<NZ, -1>(e, lds + team * NZ, tw, t);
+  fft::fft_slots<NZ, -1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
   if (act) {
     cx<R> *dst = P + ((size_t)q * g.Ny + (plane0 + pl)) * g.ps + (size_t)ix * g.Nz;
 #pragma unroll
