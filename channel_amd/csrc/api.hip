@@ -76,12 +76,19 @@ __global__ __launch_bounds__(TT * 8) void k_to_host(const cx<double> *__restrict
   }
 }
 
-// (0,0) columns of all three components: out[c][row]
-__global__ void k_gather00(const cx<double> *__restrict__ V, cx<double> *__restrict__ out, Geom g) {
+// outstats staging: the (0,0) columns of all three components as out[c][row],
+// then a copy of the device scalars; the running CFL maximum restarts at 0.
+__global__ void k_gather00(const cx<double> *__restrict__ V, cx<double> *__restrict__ out, DevScalars *sc, Geom g) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) {
+    DevScalars *stash = reinterpret_cast<DevScalars *>(out + 3 * g.Ny);
+    *stash = *sc;
+    sc->cfl_bits = 0ull;
+  }
   if (r >= g.Ny) return;
   for (int c = 0; c < 3; c++) out[c * g.Ny + r] = V[((size_t)c * g.Ny + r) * g.ps + g.nz];
 }
+constexpr size_t STASH = (sizeof(DevScalars) + sizeof(cx<double>) - 1) / sizeof(cx<double>);
 
 // grid-stride streaming copy of n 32-byte elements
 __global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ a, double4 *__restrict__ b, size_t n) {
@@ -117,13 +124,12 @@ struct chan_ctx {
   Geom g;
   hipStream_t st = nullptr;
   // device state
-  cx<double> *V = nullptr, *old = nullptr, *rhs = nullptr, *ucor = nullptr, *col00 = nullptr;
+  cx<double> *V = nullptr, *old = nullptr, *rhs = nullptr, *col00 = nullptr;
   double2 *L = nullptr;
   cx<double> *A = nullptr, *B = nullptr, *P = nullptr;  // FP64 transform intermediates
   cx<double> *twz = nullptr, *twx = nullptr;
-  double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr, *cflpart = nullptr;
+  double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr;
   DevScalars *sc = nullptr;
-  size_t ncflpart = 0;
   // host scalars
   double time = 0, deltat = 0;
   bool field_set = false;
@@ -251,7 +257,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   g.nxd = h.nxd;
   g.nzd = h.nzd;
   g.npair = g.Nx * g.Nz;
-  g.ps = ((size_t)g.npair + 63) / 64 * 64;
+  g.ps = ((size_t)g.npair + 1 + 63) / 64 * 64;  // one spare slot: the ucor solve
   g.alfa0 = p->alfa0;
   g.beta0 = p->beta0;
   c->time = p->time;
@@ -269,8 +275,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if ((rc = dalloc(&c->old, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->rhs, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->L, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
-  if ((rc = dalloc(&c->ucor, (size_t)g.Ny))) return bail(rc);
-  if ((rc = dalloc(&c->col00, 3 * (size_t)g.Ny))) return bail(rc);
+  if ((rc = dalloc(&c->col00, 3 * (size_t)g.Ny + STASH))) return bail(rc);
   if ((rc = dalloc(&c->A, 3 * (size_t)g.Ny * slab))) return bail(rc);
   if ((rc = dalloc(&c->B, 6 * (size_t)g.Ny * slab))) return bail(rc);
   if ((rc = dalloc(&c->P, 6 * NyP))) return bail(rc);
@@ -281,8 +286,6 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if ((rc = dalloc(&c->y, (size_t)g.Ny))) return bail(rc);
   if ((rc = dalloc(&c->dy, (size_t)g.ny + 1))) return bail(rc);
   if ((rc = dalloc(&c->sc, 1))) return bail(rc);
-  c->ncflpart = (size_t)(g.nzd / ZB) * g.Ny;
-  if ((rc = dalloc(&c->cflpart, c->ncflpart))) return bail(rc);
 
   std::vector<cx<double>> tw;
   twiddles(tw, g.nzd);
@@ -301,8 +304,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if (hipMemcpy(c->sc, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
   // stored explicit terms start at zero (ODE3 = 0 in the first substep)
   if (hipMemset(c->old, 0, sizeof(cx<double>) * 2 * (g.ny - 1) * plane) != hipSuccess ||
-      hipMemset(c->V, 0, sizeof(cx<double>) * 3 * NyP) != hipSuccess ||
-      hipMemset(c->ucor, 0, sizeof(cx<double>) * g.Ny) != hipSuccess)
+      hipMemset(c->V, 0, sizeof(cx<double>) * 3 * NyP) != hipSuccess)
     return bail(fail(CHAN_E_STATE, "hipMemset failed"));
   *out = c;
   return 0;
@@ -316,8 +318,8 @@ int chan_destroy(chan_ctx *c) {
     hipEventDestroy(r.b);
   }
   for (auto e : c->pool) hipEventDestroy(e);
-  void *bufs[] = {c->V, c->old, c->rhs, c->L, c->ucor, c->col00, c->A, c->B, c->P, c->twz, c->twx,
-                  c->der, c->d0fac, c->y, c->dy, c->sc, c->cflpart};
+  void *bufs[] = {c->V, c->old, c->rhs, c->L, c->col00, c->A, c->B, c->P, c->twz, c->twx,
+                  c->der, c->d0fac, c->y, c->dy, c->sc};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (c->st) hipStreamDestroy(c->st);
@@ -450,15 +452,11 @@ static int products(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
   if ((rc = timed(c, "zinv", [&] { return launch_zinv<double>(g.nzd, c->st, c->V, c->A, c->twz, g, plane0, nplanes); }))) return rc;
   if ((rc = timed(c, prods ? "xpass" : "xpass_cfl", [&] {
          return launch_xpass<double>(2 * g.nxd, c->st, c->A, c->B, c->twx, g, plane0, nplanes, prods, cfl, c->dy, h.dx,
-                                     h.dz, h.factor, c->cflpart);
+                                     h.dz, h.factor, &c->sc->cfl_bits);
        })))
     return rc;
   if (prods)
     if ((rc = timed(c, "zfwd", [&] { return launch_zfwd<double>(g.nzd, c->st, c->B, c->P, c->twz, g, plane0, nplanes); }))) return rc;
-  if (cfl) {
-    const int n = (g.nzd / ZB) * nplanes;
-    if ((rc = timed(c, "cfl_reduce", [&] { return launch_cfl_reduce(c->st, c->cflpart, n, c->sc); }))) return rc;
-  }
   return 0;
 }
 
@@ -494,7 +492,6 @@ int chan_linsolve(chan_ctx *c, double lambda) {
   s.V = c->V;
   s.rhs = c->rhs;
   s.L = c->L;
-  s.ucor = c->ucor;
   s.der = c->der;
   std::memcpy(s.bc, c->hs.bc, sizeof s.bc);
   s.g = g;
@@ -505,7 +502,6 @@ int chan_linsolve(chan_ctx *c, double lambda) {
   if ((rc = timed(c, "solve_ve", [&] { return launch_solve(c->st, s); }))) return rc;
   MeanArgs m;
   m.V = c->V;
-  m.ucor = c->ucor;
   m.y = c->y;
   m.sc = c->sc;
   std::memcpy(m.bc, c->hs.bc, sizeof m.bc);
@@ -550,19 +546,17 @@ int chan_initial(chan_ctx *c) {
 int chan_outstats(chan_ctx *c, double row[11]) {
   if (!c || !row) return fail(CHAN_E_ARG, "null argument");
   const Geom &g = c->g;
-  std::vector<cx<double>> col(3 * (size_t)g.Ny);
+  std::vector<cx<double>> col(3 * (size_t)g.Ny + STASH);
   DevScalars d;
-  hipLaunchKernelGGL(k_gather00, dim3((g.Ny + 127) / 128), dim3(128), 0, c->st, c->V, c->col00, g);
+  // one kernel stages the columns and scalars and resets the running CFL
+  // maximum (cfl=0 in outstats); one copy brings them back
+  hipLaunchKernelGGL(k_gather00, dim3((g.Ny + 127) / 128), dim3(128), 0, c->st, c->V, c->col00, c->sc, g);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(col.data(), c->col00, col.size() * sizeof(col[0]), hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(&d, c->sc, sizeof d, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  std::memcpy(&d, col.data() + 3 * (size_t)g.Ny, sizeof d);
   const double rg = cfl_value(d);
   if (!std::isfinite(rg)) return fail(CHAN_E_NOTFINITE, "CFL maximum is not finite (%g): the field holds NaN or Inf", rg);
-  // reset the running maximum (cfl=0 in outstats)
-  unsigned long long zero = 0;
-  HIPCHK(hipMemcpyAsync(&c->sc->cfl_bits, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
   if (c->prm.cflmax > 0) c->deltat = c->prm.cflmax / rg;
   const double *b140 = c->hs.bnd[B_D140], *b14n = c->hs.bnd[B_D14N];
   const int ny = g.ny;

@@ -77,10 +77,10 @@ struct RhsArgs {
 };
 
 struct SolveArgs {
-  cx<double> *V;           // [3][Ny][ps]; components 1,2 receive eta and v
+  cx<double> *V;           // [3][Ny][ps]; components 1,2 receive eta and v, pair
+                           // slot npair of component 1 the ucor response
   const cx<double> *rhs;   // [2][ny-1][ps]
   double2 *L;              // [2][ny-1][ps] lower factors of the UL decomposition
-  cx<double> *ucor;        // [Ny] unit-forcing response of the (0,0) eta system
   const double *der;
   double bc[8][5];
   Geom g;
@@ -89,7 +89,6 @@ struct SolveArgs {
 
 struct MeanArgs {
   cx<double> *V;
-  const cx<double> *ucor;
   const double *y;
   DevScalars *sc;
   double bc[8][5];
@@ -114,7 +113,7 @@ hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, co
 template <typename R>
 hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
                         int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
-                        double *cfl_part);
+                        unsigned long long *cfl_bits);
 template <typename R>
 hipError_t launch_zfwd(int NZ, hipStream_t st, const cx<R> *B, cx<R> *P, const cx<R> *tw, const Geom &g, int plane0,
                        int nplanes);
@@ -127,7 +126,6 @@ hipError_t launch_solve(hipStream_t st, const SolveArgs &a);
 hipError_t launch_mean(hipStream_t st, const MeanArgs &a);
 hipError_t launch_uw(hipStream_t st, const UwArgs &a);
 hipError_t launch_flowrate(hipStream_t st, const MeanArgs &a);
-hipError_t launch_cfl_reduce(hipStream_t st, const double *part, int n, DevScalars *sc);
 hipError_t launch_to_device_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g);
 hipError_t launch_to_host_layout(hipStream_t st, const cx<double> *in, cx<double> *out, const Geom &g);
 

@@ -65,7 +65,7 @@ constexpr bool XBIG = true;
 constexpr int XXCH = fft::XCH_CPLX | fft::XCH_PAD;
 
 template <int NT>
-__device__ __forceinline__ void block_max_store(double m, double *red, double *out) {
+__device__ __forceinline__ void block_max_atomic(double m, double *red, unsigned long long *out) {
   red[threadIdx.x] = m;
   __syncthreads();
 #pragma unroll
@@ -77,7 +77,9 @@ __device__ __forceinline__ void block_max_store(double m, double *red, double *o
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  // values are >= 0 or NaN, so their bit patterns order as unsigned
+  // integers (NaN above +Inf above every finite value)
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
 }
 
 // value of the team-mate holding element M-k of a transform in slot layout:
@@ -131,7 +133,7 @@ template <int M, typename R, bool PRODUCTS>
 __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
-                                                                double dz, R factor, double *__restrict__ cfl_part) {
+                                                                double dz, R factor, unsigned long long *__restrict__ cfl_bits) {
   using PL = fft::Plan<M>;
   constexpr int T = PL::T, E = PL::E, NT = ZB * T;
   constexpr int MO = xmodes<E>();
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
       const double c = fabs((double)u[s]) / dx + fabs((double)v[s]) / dyv + fabs((double)w[s]) / dz;
       m = (c > m || c != c) ? c : m;
     }
-    block_max_store<NT>(m, red, cfl_part + blockIdx.x);
+    block_max_atomic<NT>(m, red, cfl_bits);
   }
 
   if constexpr (PRODUCTS) {
@@ -367,7 +369,7 @@ hipError_t launch_zinv(int NZ, hipStream_t st, const cx<double> *V, cx<R> *A, co
 template <typename R>
 hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
                         int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
-                        double *cfl_part) {
+                        unsigned long long *cfl_bits) {
   dim3 grid((unsigned)((g.nzd / ZB) * nplanes));
   switch (M) {
 #define X(N)                                                                                                          \
@@ -375,10 +377,10 @@ hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const c
     dim3 blk(ZB * fft::Plan<N>::T);                                                                                   \
     if (products)                                                                                                     \
       hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,    \
-                         cfl_part);                                                                                   \
+                         cfl_bits);                                                                                   \
     else                                                                                                              \
       hipLaunchKernelGGL((k_xpass<N, R, false>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,   \
-                         cfl_part);                                                                                   \
+                         cfl_bits);                                                                                   \
     return hipGetLastError();                                                                                         \
   }
     CHAN_FOR_SIZES(X)
@@ -436,7 +438,7 @@ template hipError_t launch_zinv<double>(int, hipStream_t, const cx<double> *, cx
                                         const Geom &, int, int);
 template hipError_t launch_xpass<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
                                          const Geom &, int, int, bool, int, const double *, double, double, double,
-                                         double *);
+                                         unsigned long long *);
 template hipError_t launch_zfwd<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
                                         const Geom &, int, int);
 

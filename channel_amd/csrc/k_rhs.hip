@@ -16,7 +16,10 @@ __device__ __forceinline__ cx<double> imul(double a, cx<double> X) { return cx<d
 __device__ __forceinline__ cx<double> neg(cx<double> X) { return cx<double>{-X.x, -X.y}; }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restrict__ der,
+                                             const DevScalars *__restrict__ sc) {
+  // der and sc are separate restrict arguments so that their wave-uniform
+  // values come through the scalar cache instead of vector registers.
   const Geom g = a.g;
   const int p = blockIdx.x * 256 + threadIdx.x;
   if (p >= g.npair) return;
@@ -28,13 +31,15 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
   const int y1 = min(g.ny - 1, y0 + a.chunk - 1);
   const size_t ps = g.ps, pstride = (size_t)g.Ny * ps;
   const bool mean = (ix == 0 && iz == 0);
-  const double meanpx = a.sc->meanpx, meanpz = a.sc->meanpz;
+  const double meanpx = sc->meanpx, meanpz = sc->meanpz;
 
-  // window slot j holds row iy-2+j; plane index of row r is r+1
-  cx<double> wp[6][5], wv[3][5];
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const size_t off = (size_t)(y0 - 1 + j) * ps + p;
+  // window slot j holds row iy-2+j; plane index of row r is r+1.  The stored
+  // explicit terms of row iy are loaded one iteration early into nxo,
+  // unconditionally (clamped row), so the loads overlap the previous row.
+  const size_t rs = (size_t)(g.ny - 1) * ps;  // component stride of old/rhs
+  cx<double> wp[6][5], wv[3][5], nxo[2];
+  auto load_row = [&](int r, int j) {
+    const size_t off = (size_t)(r + 1) * ps + p;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       const cx<R> v = a.P[q * pstride + off];
@@ -42,21 +47,21 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) wv[c][j] = a.V[c * pstride + off];
-  }
+  };
+#pragma unroll
+  for (int j = 0; j < 4; j++) load_row(y0 - 2 + j, j);
+  nxo[0] = a.old[(size_t)(y0 - 1) * ps + p];
+  nxo[1] = a.old[rs + (size_t)(y0 - 1) * ps + p];
 
-  const size_t rs = (size_t)(g.ny - 1) * ps;  // component stride of old/rhs
   for (int iy = y0; iy <= y1; iy++) {
+    load_row(iy + 2, 4);
+    const cx<double> old_eta = nxo[0], old_v = nxo[1];
     {
-      const size_t off = (size_t)(iy + 3) * ps + p;
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        const cx<R> v = a.P[q * pstride + off];
-        wp[q][4] = cx<double>{(double)v.x, (double)v.y};
-      }
-#pragma unroll
-      for (int c = 0; c < 3; c++) wv[c][4] = a.V[c * pstride + off];
+      const int on = min(iy, g.ny - 2);
+      nxo[0] = a.old[(size_t)on * ps + p];
+      nxo[1] = a.old[rs + (size_t)on * ps + p];
     }
-    const double *d = a.der + (size_t)(iy - 1) * 20;  // d0[5] d1[5] d2[5] d4[5]
+    const double *d = der + (size_t)(iy - 1) * 20;  // d0[5] d1[5] d2[5] d4[5]
     auto DD = [&](int f, int q) {
       cx<double> s = scl(d[f * 5 + 0], wp[q][0]);
 #pragma unroll
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
         const double os = a.ni * (d[15 + j] - 2.0 * k2 * d[10 + j] + k2 * k2 * d[j]);
         impl = add(impl, scl(os, wv[1][j]));
       }
-      const cx<double> old = a.old[rs + o];
+      const cx<double> old = old_v;
       const cx<double> un = scl(a.ode0, unkn);
       const cx<double> r = sub(add(add(cx<double>{un.x / a.dt, un.y / a.dt}, impl), scl(a.ode1, expl)), scl(a.ode2, old));
       a.old[rs + o] = expl;
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
           impl = add(impl, scl(sq, sub(imul(be, wv[0][j]), imul(al, wv[2][j]))));
         }
       }
-      const cx<double> old = a.old[o];
+      const cx<double> old = old_eta;
       const cx<double> un = scl(a.ode0, unkn);
       const cx<double> r = sub(add(add(cx<double>{un.x / a.dt, un.y / a.dt}, impl), scl(a.ode1, ex)), scl(a.ode2, old));
       a.old[o] = ex;
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a) {
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a) {
   dim3 grid((unsigned)((a.g.npair + 255) / 256), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
-  hipLaunchKernelGGL((k_rhs<R>), grid, dim3(256), 0, st, a);
+  hipLaunchKernelGGL((k_rhs<R>), grid, dim3(256), 0, st, a, a.der, a.sc);
   return hipGetLastError();
 }
 
