@@ -62,10 +62,17 @@ def algorithmic_bytes(cfg, nxd, nzd, c=16):
     return per, step
 
 
-def cpu_baseline(cfg, V, nplanes=32, nix=12):
+def cpu_baseline(cfg, V, nplanes=None, nix=None):
     """The oracle (plain C restatement of the reference step, OpenMP over
     planes and x modes) timed on the host cores on a bounded sample of the
-    same workload, extrapolated to RK3 steps/s."""
+    same workload, extrapolated to RK3 steps/s.  By default the sample is
+    nearly one whole substep (products of the ny-1 interior planes, the rhs
+    of rows 3..ny-3, the solves of the x-mode columns 1..nx): about 10 s on
+    16 cores at 191x384x189."""
+    if nplanes is None:
+        nplanes = cfg.ny - 1
+    if nix is None:
+        nix = cfg.nx
     from oracle import oracle as orc
     threads = min(16, os.cpu_count() or 1)
     orc.set_threads(threads)

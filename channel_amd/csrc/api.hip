@@ -95,11 +95,12 @@ __global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ a, dou
   for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = a[i];
 }
 
-void twiddles(std::vector<cx<double>> &w, int n) {
+template <typename T>
+void twiddles(std::vector<cx<T>> &w, int n) {
   w.resize(n);
   for (int e = 0; e < n; e++) {
     const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
-    w[e] = cx<double>{(double)cosl(ang), (double)sinl(ang)};
+    w[e] = cx<T>{(T)cosl(ang), (T)sinl(ang)};
   }
 }
 
@@ -126,9 +127,12 @@ struct chan_ctx {
   // device state
   cx<double> *V = nullptr, *old = nullptr, *rhs = nullptr, *col00 = nullptr;
   double2 *L = nullptr;
-  cx<double> *A = nullptr, *B = nullptr, *P = nullptr;  // FP64 transform intermediates
-  cx<double> *twz = nullptr, *twx = nullptr;
+  // transform intermediates and twiddles: complex FP64, or complex FP32 in
+  // the split mode
+  bool split = false;
+  void *A = nullptr, *B = nullptr, *P = nullptr, *twz = nullptr, *twx = nullptr;
   double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr;
+  double *stat = nullptr;  // statistics: [3][ny+1] mean in, [7][ny+1] profiles out
   DevScalars *sc = nullptr;
   // host scalars
   double time = 0, deltat = 0;
@@ -205,6 +209,12 @@ int timed(chan_ctx *c, const char *name, F launch) {
   return 0;
 }
 
+int dalloc_bytes(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes + 256);
+  if (e != hipSuccess) return fail((int)e, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+  return 0;
+}
+
 template <class T>
 int dalloc(T **p, size_t n) {
   hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 256);
@@ -229,7 +239,8 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if (!p || !out) return fail(CHAN_E_ARG, "null argument");
   *out = nullptr;
   if (p->nx < 1 || p->nz < 1 || p->ny < 8) return fail(CHAN_E_ARG, "need nx >= 1, nz >= 1, ny >= 8 (got %d %d %d)", p->nx, p->ny, p->nz);
-  if (p->precision != CHAN_FP64) return fail(CHAN_E_ARG, "precision mode %d is not available", p->precision);
+  if (p->precision != CHAN_FP64 && p->precision != CHAN_SPLIT)
+    return fail(CHAN_E_ARG, "precision mode %d is not available", p->precision);
   if (p->cpi && p->cpi_type != 0 && p->cpi_type != 1) return fail(CHAN_E_CPITYPE, "Wrong selection of CPI_Type (%d)", p->cpi_type);
   if (!(p->alfa0 > 0) || !(p->beta0 > 0) || !(p->ni > 0) || !(p->ymax > p->ymin) || !(p->a > 0))
     return fail(CHAN_E_ARG, "alfa0, beta0, ni, a must be positive and ymax > ymin");
@@ -237,6 +248,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CHAN_E_NODEV, "no HIP device");
   chan_ctx *c = new chan_ctx();
   c->prm = *p;
+  c->split = p->precision == CHAN_SPLIT;
   setup_build(c->hs, p->nx, p->ny, p->nz, p->alfa0, p->beta0, p->a, p->ymin, p->ymax);
   const HostSetup &h = c->hs;
   // the x pass holds at most (E+2)/3 x modes per thread, i.e. Nx <= 2nxd/3,
@@ -276,22 +288,30 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if ((rc = dalloc(&c->rhs, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->L, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->col00, 3 * (size_t)g.Ny + STASH))) return bail(rc);
-  if ((rc = dalloc(&c->A, 3 * (size_t)g.Ny * slab))) return bail(rc);
-  if ((rc = dalloc(&c->B, 6 * (size_t)g.Ny * slab))) return bail(rc);
-  if ((rc = dalloc(&c->P, 6 * NyP))) return bail(rc);
-  if ((rc = dalloc(&c->twz, (size_t)g.nzd))) return bail(rc);
-  if ((rc = dalloc(&c->twx, (size_t)2 * g.nxd))) return bail(rc);
+  const size_t esz = c->split ? sizeof(cx<float>) : sizeof(cx<double>);
+  // B doubles as the staging area of the field and rhs transposes, which
+  // needs 3*Ny*npair complex FP64 <= 6*Ny*slab*esz
+  if ((rc = dalloc_bytes(&c->A, 3 * (size_t)g.Ny * slab * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->B, 6 * (size_t)g.Ny * slab * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->P, 6 * NyP * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->twz, (size_t)g.nzd * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->twx, (size_t)2 * g.nxd * esz))) return bail(rc);
   if ((rc = dalloc(&c->der, (size_t)(g.ny - 1) * 20))) return bail(rc);
   if ((rc = dalloc(&c->d0fac, (size_t)(g.ny + 1) * 5))) return bail(rc);
   if ((rc = dalloc(&c->y, (size_t)g.Ny))) return bail(rc);
   if ((rc = dalloc(&c->dy, (size_t)g.ny + 1))) return bail(rc);
   if ((rc = dalloc(&c->sc, 1))) return bail(rc);
+  if ((rc = dalloc(&c->stat, (size_t)10 * (g.ny + 1)))) return bail(rc);
 
-  std::vector<cx<double>> tw;
-  twiddles(tw, g.nzd);
-  if (hipMemcpy(c->twz, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
-  twiddles(tw, 2 * g.nxd);
-  if (hipMemcpy(c->twx, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
+  auto upload_tw = [&](auto proto) {
+    std::vector<decltype(proto)> tw;
+    twiddles(tw, g.nzd);
+    if (hipMemcpy(c->twz, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    twiddles(tw, 2 * g.nxd);
+    if (hipMemcpy(c->twx, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    return 0;
+  };
+  if (c->split ? upload_tw(cx<float>{}) : upload_tw(cx<double>{})) return bail(fail(CHAN_E_STATE, "upload failed"));
   if (hipMemcpy(c->der, h.der, sizeof(double) * 20 * (g.ny - 1), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d0fac, h.d0fac, sizeof(double) * 5 * (g.ny + 1), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->y, h.y, sizeof(double) * g.Ny, hipMemcpyHostToDevice) != hipSuccess ||
@@ -319,7 +339,7 @@ int chan_destroy(chan_ctx *c) {
   }
   for (auto e : c->pool) hipEventDestroy(e);
   void *bufs[] = {c->V, c->old, c->rhs, c->L, c->col00, c->A, c->B, c->P, c->twz, c->twx,
-                  c->der, c->d0fac, c->y, c->dy, c->sc};
+                  c->der, c->d0fac, c->y, c->dy, c->sc, c->stat};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (c->st) hipStreamDestroy(c->st);
@@ -371,8 +391,9 @@ int chan_setup_host(const chan_params *p, int32_t sizes[5], double *y, double *d
 int chan_set_field(chan_ctx *c, const double *V) {
   if (!c || !V) return fail(CHAN_E_ARG, "null argument");
   const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
-  HIPCHK(hipMemcpyAsync(c->B, V, n * sizeof(cx<double>), hipMemcpyHostToDevice, c->st));
-  HIPCHK(launch_to_device_layout(c->st, c->B, c->V, c->g));
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
+  HIPCHK(hipMemcpyAsync(stage, V, n * sizeof(cx<double>), hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_to_device_layout(c->st, stage, c->V, c->g));
   HIPCHK(hipStreamSynchronize(c->st));
   c->field_set = true;
   return 0;
@@ -381,8 +402,9 @@ int chan_set_field(chan_ctx *c, const double *V) {
 int chan_get_field(chan_ctx *c, double *V) {
   if (!c || !V) return fail(CHAN_E_ARG, "null argument");
   const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
-  HIPCHK(launch_to_host_layout(c->st, c->V, c->B, c->g));
-  HIPCHK(hipMemcpyAsync(V, c->B, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
+  HIPCHK(launch_to_host_layout(c->st, c->V, stage, c->g));
+  HIPCHK(hipMemcpyAsync(V, stage, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -392,7 +414,7 @@ int chan_get_rhs(chan_ctx *c, double *R) {
   // rhs is [2][ny-1][ps]: the same transpose as the field with ny-1 rows
   Geom g = c->g;
   g.Ny = g.ny - 1;
-  cx<double> *stage = c->B;
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
   for (int k = 0; k < 2; k++) {
     Geom g1 = g;
     // one component at a time: the transpose kernel runs over 3 components,
@@ -445,20 +467,50 @@ int chan_set_scalars(chan_ctx *c, const chan_scalars *s) {
   return 0;
 }
 
-static int products(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
+extern "C++" {
+// K1-K3 with transform arithmetic and intermediates in R (double, or float
+// in the split mode)
+template <typename R>
+static int products_t(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
   const Geom &g = c->g;
   const HostSetup &h = c->hs;
+  cx<R> *A = static_cast<cx<R> *>(c->A), *B = static_cast<cx<R> *>(c->B), *P = static_cast<cx<R> *>(c->P);
+  const cx<R> *twz = static_cast<const cx<R> *>(c->twz), *twx = static_cast<const cx<R> *>(c->twx);
   int rc;
-  if ((rc = timed(c, "zinv", [&] { return launch_zinv<double>(g.nzd, c->st, c->V, c->A, c->twz, g, plane0, nplanes); }))) return rc;
+  if ((rc = timed(c, "zinv", [&] { return launch_zinv<R>(g.nzd, c->st, c->V, A, twz, g, plane0, nplanes); }))) return rc;
   if ((rc = timed(c, prods ? "xpass" : "xpass_cfl", [&] {
-         return launch_xpass<double>(2 * g.nxd, c->st, c->A, c->B, c->twx, g, plane0, nplanes, prods, cfl, c->dy, h.dx,
-                                     h.dz, h.factor, &c->sc->cfl_bits);
+         return launch_xpass<R>(2 * g.nxd, c->st, A, B, twx, g, plane0, nplanes, prods, cfl, c->dy, h.dx, h.dz,
+                                (R)h.factor, &c->sc->cfl_bits);
        })))
     return rc;
   if (prods)
-    if ((rc = timed(c, "zfwd", [&] { return launch_zfwd<double>(g.nzd, c->st, c->B, c->P, c->twz, g, plane0, nplanes); }))) return rc;
+    if ((rc = timed(c, "zfwd", [&] { return launch_zfwd<R>(g.nzd, c->st, B, P, twz, g, plane0, nplanes); }))) return rc;
   return 0;
 }
+
+static int products(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
+  return c->split ? products_t<float>(c, plane0, nplanes, prods, cfl) : products_t<double>(c, plane0, nplanes, prods, cfl);
+}
+
+template <typename R>
+static int rhs_t(chan_ctx *c, const double ode[3]) {
+  RhsArgs<R> a;
+  a.P = static_cast<const cx<R> *>(c->P);
+  a.V = c->V;
+  a.old = c->old;
+  a.rhs = c->rhs;
+  a.der = c->der;
+  a.sc = c->sc;
+  a.g = c->g;
+  a.ode0 = ode[0];
+  a.ode1 = ode[1];
+  a.ode2 = ode[2];
+  a.dt = c->deltat;
+  a.ni = c->prm.ni;
+  a.chunk = 48;
+  return timed(c, "rhs", [&] { return launch_rhs<R>(c->st, a); });
+}
+}  // extern "C++"
 
 int chan_buildrhs(chan_ctx *c, const double ode[3], int32_t compute_cfl) {
   if (!c || !ode) return fail(CHAN_E_ARG, "null argument");
@@ -466,21 +518,7 @@ int chan_buildrhs(chan_ctx *c, const double ode[3], int32_t compute_cfl) {
   const Geom &g = c->g;
   int rc;
   if ((rc = products(c, 0, g.Ny, true, compute_cfl ? 1 : 0))) return rc;
-  RhsArgs<double> a;
-  a.P = c->P;
-  a.V = c->V;
-  a.old = c->old;
-  a.rhs = c->rhs;
-  a.der = c->der;
-  a.sc = c->sc;
-  a.g = g;
-  a.ode0 = ode[0];
-  a.ode1 = ode[1];
-  a.ode2 = ode[2];
-  a.dt = c->deltat;
-  a.ni = c->prm.ni;
-  a.chunk = 48;
-  return timed(c, "rhs", [&] { return launch_rhs<double>(c->st, a); });
+  return c->split ? rhs_t<float>(c, ode) : rhs_t<double>(c, ode);
 }
 
 int chan_linsolve(chan_ctx *c, double lambda) {
@@ -541,6 +579,23 @@ int chan_initial(chan_ctx *c) {
   m.cpi = c->prm.cpi;
   m.cpi_type = c->prm.cpi_type;
   return timed(c, "flowrate", [&] { return launch_flowrate(c->st, m); });
+}
+
+int chan_stats(chan_ctx *c, const double *avg, double *out) {
+  if (!c || !out) return fail(CHAN_E_ARG, "null argument");
+  if (!c->field_set) return fail(CHAN_E_STATE, "stats before set_field");
+  const Geom &g = c->g;
+  const size_t n1 = (size_t)g.ny + 1;
+  double *dav = nullptr;
+  if (avg) {
+    dav = c->stat;
+    HIPCHK(hipMemcpyAsync(dav, avg, sizeof(double) * 3 * n1, hipMemcpyHostToDevice, c->st));
+  }
+  int rc;
+  if ((rc = timed(c, "stats", [&] { return launch_stats(c->st, c->V, dav, c->stat + 3 * n1, g); }))) return rc;
+  HIPCHK(hipMemcpyAsync(out, c->stat + 3 * n1, sizeof(double) * 7 * n1, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
 }
 
 int chan_outstats(chan_ctx *c, double row[11]) {

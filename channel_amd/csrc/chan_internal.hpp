@@ -80,7 +80,8 @@ struct SolveArgs {
   cx<double> *V;           // [3][Ny][ps]; components 1,2 receive eta and v, pair
                            // slot npair of component 1 the ucor response
   const cx<double> *rhs;   // [2][ny-1][ps]
-  double2 *L;              // [2][ny-1][ps] lower factors of the UL decomposition
+  double2 *L;              // [2][ny-1][gs] lower factors of the UL decomposition, one
+                           // row per (kx, |kz|) group, gs <= ps
   const double *der;
   double bc[8][5];
   Geom g;
@@ -122,6 +123,7 @@ hipError_t launch_fft_bench(int N, int sign, int variant, const cx<double> *in, 
                             int batch, int inner);
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a);
+hipError_t launch_stats(hipStream_t st, const cx<double> *V, const double *avg, double *out, const Geom &g);
 hipError_t launch_solve(hipStream_t st, const SolveArgs &a);
 hipError_t launch_mean(hipStream_t st, const MeanArgs &a);
 hipError_t launch_uw(hipStream_t st, const UwArgs &a);

@@ -145,20 +145,42 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   const int iy = plane0 + pl - 1;
   R *fr = fl + zi * fft::lds_reals<M, XXCH>();
 
+  // twiddles in LDS: their reads then wait on the LDS counter, not behind
+  // the row loads still in flight on the vector-memory counter
+  __shared__ __align__(16) cx<R> twl[M];
+  for (int i = tid; i < M; i += NT) twl[i] = tw[i];
+  __syncthreads();
+
   // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
   // (k <= nx) or the conjugate of mode M-k (k >= M-nx); Im of mode 0 is
-  // dropped as c2r does.
-  auto load_ext = [&](int c, cx<R>(&e)[E]) {
+  // dropped as c2r does.  All loads of a component are issued first
+  // (unconditionally, from clamped in-row indices; slots that are zero for
+  // every lane are skipped by a wave-uniform test), the conjugation and
+  // zeroing follow as selects, so the loads overlap one another.
+  auto load_raw = [&](int c, cx<R>(&e)[E]) {
     const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB + zi;
 #pragma unroll
     for (int s = 0; s < E; s++) {
+      if (T * s > nx && T * s + T - 1 < M - nx) {
+        e[s] = cx<R>{0, 0};
+      } else {
+        const int k = t + T * s;
+        const int m = k <= nx ? k : (k >= M - nx ? M - k : 0);
+        e[s] = src[(size_t)m * ZB];
+      }
+    }
+  };
+  auto fix_ext = [&](cx<R>(&e)[E]) {
+#pragma unroll
+    for (int s = 0; s < E; s++) {
       const int k = t + T * s;
-      cx<R> v = {0, 0};
-      if (k <= nx) {
-        v = src[(size_t)k * ZB];
-        if (k == 0) v.y = 0;
-      } else if (k >= M - nx) {
-        v = conj(src[(size_t)(M - k) * ZB]);
+      cx<R> v = e[s];
+      if (k == 0) v.y = 0;
+      if (k > nx) {
+        if (k >= M - nx)
+          v.y = -v.y;
+        else
+          v = cx<R>{0, 0};
       }
       e[s] = v;
     }
@@ -167,18 +189,21 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   R u[E], v[E], w[E];
   {
     cx<R> e[E], f[E];
-    load_ext(0, e);
-    load_ext(1, f);
+    load_raw(0, e);
+    load_raw(1, f);
+    fix_ext(e);
+    fix_ext(f);
 #pragma unroll
     for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
-    fft::fft_slots<M, +1, XBIG, XXCH>(e, fr, tw, t);
+    load_raw(2, f);  // in flight during the first transform
+    fft::fft_slots<M, +1, XBIG, XXCH>(e, fr, twl, t);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       u[s] = e[s].x;
       v[s] = e[s].y;
     }
-    load_ext(2, f);
-    fft::fft_slots<M, +1, XBIG, XXCH>(f, fr, tw, t);
+    fix_ext(f);
+    fft::fft_slots<M, +1, XBIG, XXCH>(f, fr, twl, t);
 #pragma unroll
     for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
         }
         e[s] = cx<R>{a, b};
       }
-      fft::fft_slots<M, -1, XBIG, XXCH>(e, fr, tw, t);
+      fft::fft_slots<M, -1, XBIG, XXCH>(e, fr, twl, t);
       // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
       cx<R> mz[MO];
       mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
@@ -441,5 +466,12 @@ template hipError_t launch_xpass<double>(int, hipStream_t, const cx<double> *, c
                                          unsigned long long *);
 template hipError_t launch_zfwd<double>(int, hipStream_t, const cx<double> *, cx<double> *, const cx<double> *,
                                         const Geom &, int, int);
+template hipError_t launch_zinv<float>(int, hipStream_t, const cx<double> *, cx<float> *, const cx<float> *,
+                                       const Geom &, int, int);
+template hipError_t launch_xpass<float>(int, hipStream_t, const cx<float> *, cx<float> *, const cx<float> *,
+                                        const Geom &, int, int, bool, int, const double *, double, double, float,
+                                        unsigned long long *);
+template hipError_t launch_zfwd<float>(int, hipStream_t, const cx<float> *, cx<float> *, const cx<float> *,
+                                       const Geom &, int, int);
 
 }  // namespace chan

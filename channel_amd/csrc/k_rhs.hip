@@ -154,5 +154,6 @@ hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a) {
 }
 
 template hipError_t launch_rhs<double>(hipStream_t, const RhsArgs<double> &);
+template hipError_t launch_rhs<float>(hipStream_t, const RhsArgs<float> &);
 
 }  // namespace chan

@@ -2,7 +2,10 @@
 // (linsolve, linsolve_blocking.inc:3-107) as three kernels:
 //
 //   k_solve_ve : one lane per (pair, system, real/imaginary part); system 0 =
-//                eta, 1 = v.  Builds the pentadiagonal rows lambda*D - nu*L
+//                eta, 1 = v.  The pairs (kx, kz) and (kx, -kz) have the same
+//                k^2, hence the same matrix: their four lanes (sign x part)
+//                sit side by side and share one row of stored lower factors.
+//                Builds the pentadiagonal rows lambda*D - nu*L
 //                of etamat / D2vmat, folds the wall conditions into rows 1,2
 //                and ny-2,ny-1 (applybc_0 / applybc_n, dnsdata.f90:458-472),
 //                factorises bottom-up with reciprocal pivots fused with the
@@ -11,8 +14,9 @@
 //                (LeftLU5divStep2, :82-100) and recovers wall and ghost values
 //                (linsolve_blocking.inc:50-61).  The matrices are real, so the
 //                real and imaginary parts of a right-hand side are independent
-//                recurrences on adjacent lanes.  Both lanes store the same
-//                lower factors, so each reads back only what it wrote.
+//                recurrences on adjacent lanes.  All four lanes of a group
+//                store the same lower factors, so each reads back only what
+//                it wrote itself.
 //                The spare pair slot npair (plane padding) solves the
 //                unit-forcing response ucor: the (0,0) eta matrix with
 //                right-hand side 1 and homogeneous walls (:65-76).
@@ -47,18 +51,25 @@ __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__r
   // prefetches wait on.
   const Geom g = a.g;
   const int gid = blockIdx.x * 256 + threadIdx.x;
-  const int p = gid >> 1, part = gid & 1;  // part 0 real, 1 imaginary
-  if (p > g.npair) return;                 // p == npair: the ucor system
-  const bool spare = p == g.npair;
+  // lane = 4*group + 2*sign + part; group (ix, |iz|) for iz >= 0, part 0 real,
+  // 1 imaginary; the group after the last one solves ucor in pair slot npair
+  const int grp = gid >> 2, sgn = (gid >> 1) & 1, part = gid & 1;
+  const int ngrp = g.Nx * (g.nz + 1);
+  if (grp > ngrp) return;
+  const bool spare = grp == ngrp;
+  const int ix = spare ? 0 : grp / (g.nz + 1), iza = spare ? 0 : grp % (g.nz + 1);
+  if (sgn && (spare || iza == 0)) return;  // kz = 0 has no mirror
+  const int iz = sgn ? -iza : iza;
+  const int p = spare ? g.npair : ix * g.Nz + g.nz + iz;
   const int sys = blockIdx.y;  // 0: eta (component 1), 1: v (component 2)
   const int ny = g.ny;
-  const int ix = spare ? 0 : p / g.Nz, iz = spare ? 0 : p % g.Nz - g.nz;
+  const size_t gs = ((size_t)ngrp + 1 + 63) / 64 * 64;  // group stride of L
   const double k2 = (g.alfa0 * ix) * (g.alfa0 * ix) + (g.beta0 * iz) * (g.beta0 * iz);
   const bool mean = !spare && ix == 0 && iz == 0;
   const size_t ps = g.ps, rs = 2 * ps;  // row stride in doubles
   double *col = reinterpret_cast<double *>(a.V + (size_t)sys * g.Ny * ps + p) + part;  // row iy at col[(iy+1)*rs]
   const double *rc = reinterpret_cast<const double *>(a.rhs + (size_t)sys * (ny - 1) * ps + p) + part;
-  double2 *Lc = a.L + (size_t)sys * (ny - 1) * ps + p;
+  double2 *Lc = a.L + (size_t)sys * (ny - 1) * gs + grp;  // row iy at Lc[(iy-1)*gs]
   const double *bc0 = a.bc[sys ? BC_V0 : BC_ETA0], *bc0m1 = a.bc[sys ? BC_V0M1 : BC_ETA0M1];
   const double *bcn = a.bc[sys ? BC_VN : BC_ETAN], *bcnp1 = a.bc[sys ? BC_VNP1 : BC_ETANP1];
   // wall value of eta: (Re u - Im w) + i (Im u + Re w) with u = u0/uN, w = 0
@@ -123,7 +134,7 @@ __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__r
     // backward sweep
     const double xi = (b - (p1 * x1 + p2 * x2)) * r;
     col[(size_t)(iy + 1) * rs] = xi;
-    Lc[(size_t)(iy - 1) * ps] = make_double2(iy <= 2 ? 0.0 : A[0], iy == 1 ? 0.0 : A[1]);
+    Lc[(size_t)(iy - 1) * gs] = make_double2(iy <= 2 ? 0.0 : A[0], iy == 1 ? 0.0 : A[1]);
     l2m2 = l1m2;
     l2m1 = l1m1;
     l1m2 = A[0];
@@ -168,7 +179,7 @@ __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__r
     for (int u = 0; u < PF; u++) {
       const int r = min(1 + u, ny - 1);
       xr[u] = col[(size_t)(r + 1) * rs];
-      lr[u] = Lc[(size_t)(r - 1) * ps];
+      lr[u] = Lc[(size_t)(r - 1) * gs];
     }
     int iy = 1;
     for (; iy + (PF - 1) <= ny - 1; iy += PF) {
@@ -179,7 +190,7 @@ __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__r
         const double2 L = lr[u];
         const int nr = min(r + PF, ny - 1);
         xr[u] = col[(size_t)(nr + 1) * rs];
-        lr[u] = Lc[(size_t)(nr - 1) * ps];
+        lr[u] = Lc[(size_t)(nr - 1) * gs];
         down_row(r, x, L);
       }
     }
@@ -453,7 +464,8 @@ __global__ __launch_bounds__(256) void k_flowrate(MeanArgs a) {
 
 // ------------------------------------------------------------ launchers --
 hipError_t launch_solve(hipStream_t st, const SolveArgs &a) {
-  dim3 grid((unsigned)((2 * (a.g.npair + 1) + 255) / 256), 2);
+  const size_t lanes = (size_t)4 * (a.g.Nx * (a.g.nz + 1) + 1);
+  dim3 grid((unsigned)((lanes + 255) / 256), 2);
   hipLaunchKernelGGL(k_solve_ve, grid, dim3(256), 0, st, a, a.der);
   return hipGetLastError();
 }

@@ -214,7 +214,9 @@ CONTAINS
     character(len=60) :: fn
     time = r(1); deltat = r(11)
     WRITE(*,"(F10.4,3X,4(F11.6,3X),4(F9.4,3X),2(F9.6,3X))") r
-    WRITE(101,*) r
+    ! one record per line with 17 significant digits (list-directed output
+    ! would wrap the 11 values over several lines with flang)
+    WRITE(101,"(11(1X,ES24.16E3))") r
     FLUSH(101)
     IF (dt_save > 0) THEN
       IF (((FLOOR((time+0.5*deltat)/dt_save) > FLOOR((time-0.5*deltat)/dt_save)) .AND. (istep > 1))) THEN

@@ -64,6 +64,7 @@ SIGNATURES = {
     "chan_buildrhs": (ctypes.c_int, [_P, _D, _I32]),
     "chan_linsolve": (ctypes.c_int, [_P, ctypes.c_double]),
     "chan_outstats": (ctypes.c_int, [_P, _D]),
+    "chan_stats": (ctypes.c_int, [_P, _D, _D]),
     "chan_step": (ctypes.c_int, [_P, _D]),
     "chan_steps": (ctypes.c_int, [_P, _I32, _D]),
     "chan_timing": (ctypes.c_int, [_P, _I32]),

@@ -103,6 +103,18 @@ class Channel:
         lib.check(self._L.chan_step(self.h, lib.dptr(r)))
         return r
 
+    def stats(self, avg=None):
+        """Statistics profiles of the current field for rows 0..ny: dict of
+        mean (3, ny+1), var (3, ny+1) and uv (ny+1,); avg (3, ny+1) is the mean
+        subtracted from the (0,0) mode (default: the field's own)."""
+        n1 = self.cfg.ny + 1
+        out = np.zeros((7, n1))
+        a = None
+        if avg is not None:
+            a = np.ascontiguousarray(avg, dtype=np.float64).reshape(3, n1)
+        lib.check(self._L.chan_stats(self.h, lib.dptr(a) if a is not None else None, lib.dptr(out)))
+        return {"mean": out[0:3], "var": out[3:6], "uv": out[6]}
+
     def steps(self, n):
         r = np.zeros((n, 11))
         if n:

@@ -137,6 +137,17 @@ int chan_outstats(chan_ctx *c, double row[11]);
 int chan_step(chan_ctx *c, double row[11]);
 int chan_steps(chan_ctx *c, int32_t n, double *rows);
 
+/* Wall-normal statistics of the current field for rows iy = 0..ny, the
+ * spectral sums of postpro/history/var_history.f90:36-87 (weight 1 at kx = 0,
+ * 2 otherwise), out = [7][ny+1]:
+ *   rows 0-2  plane means Re V(iy,0,0,c), c = u, v, w;
+ *   rows 3-5  variances sum f |V(iy,iz,ix,c) - d m(iy,c)|^2, d = 1 only for
+ *             the (0,0) mode;
+ *   row  6    Reynolds shear stress sum f Re[(V_u - d m_u) conj(V_v - d m_v)].
+ * m = avg[3][ny+1] (a time average, as the reference subtracts) or, when avg
+ * is NULL, the field's own plane mean (instantaneous fluctuations). */
+int chan_stats(chan_ctx *c, const double *avg, double *out);
+
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch (off by default).  chan_kernel_times fills up to cap entries of
  * names (NUL-terminated, 32 chars each), launch counts and total ms, and

@@ -78,3 +78,12 @@ def test_invalid_arguments_fail_loudly():
     with pytest.raises(lib.ChanError) as e:
         lib.setup_host(cfg)
     assert e.value.code == -1
+
+
+def test_unknown_precision_mode_is_rejected():
+    import ctypes
+    p = lib.params_from(tiny(), precision=5)
+    h = ctypes.c_void_p()
+    rc = lib.load().chan_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert b"precision" in lib.load().chan_last_error()

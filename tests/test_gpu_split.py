@@ -1,0 +1,96 @@
+"""The FP32/FP64 split step (CHAN_SPLIT) against the FP64 oracle and the FP64
+HIP step: FP32 transforms, products and product intermediates (K1-K3, the
+products read by K4); FP64 state, stencil arithmetic, solves, (0,0) mode,
+flow rates and reductions.  Tolerances are set by FP32 rounding of the
+nonlinear term: relative L2 field error <= 1e-6 after 1 and 10 steps and
+<= 1e-5 after 100, Runtimedata columns to 1e-6 of max(|value|, 1)."""
+import os
+
+import numpy as np
+import pytest
+
+from cases import config1, laminar_field, rel_l2, seeded_field, tiny
+from channel_amd import lib
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+orc.set_threads(min(16, os.cpu_count() or 1))
+SPLIT_TOL = 1e-6
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def split_channel(cfg, V):
+    from channel_amd.solver import Channel
+    g = Channel(cfg, lib.CHAN_SPLIT)
+    g.set_field(V)
+    return g
+
+
+def rows_close(rg, ro, tol):
+    scale = np.maximum(np.abs(ro), 1.0)
+    np.testing.assert_array_less(np.abs(rg - ro), tol * scale)
+
+
+def test_split_config1_vs_oracle():
+    cfg = config1()
+    say("split config1 10 steps")
+    V = seeded_field(cfg)
+    g = split_channel(cfg, V)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    errs = []
+    for k in range(10):
+        rg, ro = g.step(), o.step()
+        rows_close(rg, ro, SPLIT_TOL)
+        if k + 1 in (1, 10):
+            e = rel_l2(g.get_field(), o.get_field())
+            errs.append((k + 1, e))
+            assert e <= SPLIT_TOL, (k + 1, e)
+    print("split rel L2:", errs)
+    # the error is FP32-sized, not zero: the split path really ran in FP32
+    assert errs[0][1] > 1e-14
+
+
+def test_split_cpi_adaptive_dt():
+    cfg = tiny(nx=15, ny=48, nz=15, alfa0=0.5, beta0=1.0, ni=1 / 12431.0, cpi=True, cpi_type=1, gamma=0.161436,
+               meanflowx=0.0, cflmax=1.0, deltat=0.0)
+    say("split cpi adaptive 10 steps")
+    V = seeded_field(cfg)
+    g = split_channel(cfg, V)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    g.initial()
+    o.initial()
+    rows_close(g.outstats(), o.outstats(), SPLIT_TOL)
+    for _ in range(10):
+        rows_close(g.step(), o.step(), SPLIT_TOL)
+    assert rel_l2(g.get_field(), o.get_field()) <= SPLIT_TOL
+
+
+def test_split_100_steps_vs_fp64_gpu():
+    from channel_amd.solver import Channel
+    cfg = config1()
+    say("split vs fp64 100 steps")
+    V = seeded_field(cfg)
+    g = split_channel(cfg, V)
+    d = Channel(cfg)
+    d.set_field(V)
+    rs = g.steps(100)
+    rd = d.steps(100)
+    e = rel_l2(g.get_field(), d.get_field())
+    print("split vs fp64 after 100 steps:", e)
+    assert e <= 1e-5
+    rows_close(rs[-1], rd[-1], 1e-5)
+
+
+def test_split_laminar_fixed_point():
+    cfg = tiny(meanflowx=0.0, meanpx=2.0 / 3000.0)
+    say("split laminar")
+    V0 = laminar_field(cfg)
+    g = split_channel(cfg, V0)
+    g.steps(10)
+    assert np.max(np.abs(g.get_field() - V0)) < 1e-8
