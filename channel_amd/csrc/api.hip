@@ -97,11 +97,8 @@ __global__ __launch_bounds__(256) void k_copy(const double4 *__restrict__ a, dou
 
 template <typename T>
 void twiddles(std::vector<cx<T>> &w, int n) {
-  w.resize(n);
-  for (int e = 0; e < n; e++) {
-    const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
-    w[e] = cx<T>{(T)cosl(ang), (T)sinl(ang)};
-  }
+  w.resize((size_t)fft::tw_table_size(n));
+  fft::build_twiddles(w.data(), n);
 }
 
 }  // namespace
@@ -272,6 +269,12 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   g.ps = ((size_t)g.npair + 1 + 63) / 64 * 64;  // one spare slot: the ucor solve
   g.alfa0 = p->alfa0;
   g.beta0 = p->beta0;
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+      ncu = 256;
+    g.ncu = ncu;
+  }
   c->time = p->time;
   c->deltat = p->deltat;
 
@@ -294,8 +297,8 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   if ((rc = dalloc_bytes(&c->A, 3 * (size_t)g.Ny * slab * esz))) return bail(rc);
   if ((rc = dalloc_bytes(&c->B, 6 * (size_t)g.Ny * slab * esz))) return bail(rc);
   if ((rc = dalloc_bytes(&c->P, 6 * NyP * esz))) return bail(rc);
-  if ((rc = dalloc_bytes(&c->twz, (size_t)g.nzd * esz))) return bail(rc);
-  if ((rc = dalloc_bytes(&c->twx, (size_t)2 * g.nxd * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->twz, (size_t)fft::tw_table_size(g.nzd) * esz))) return bail(rc);
+  if ((rc = dalloc_bytes(&c->twx, (size_t)fft::tw_table_size(2 * g.nxd) * esz))) return bail(rc);
   if ((rc = dalloc(&c->der, (size_t)(g.ny - 1) * 20))) return bail(rc);
   if ((rc = dalloc(&c->d0fac, (size_t)(g.ny + 1) * 5))) return bail(rc);
   if ((rc = dalloc(&c->y, (size_t)g.Ny))) return bail(rc);
@@ -710,9 +713,9 @@ int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout) {
   cx<double> *d = nullptr, *t = nullptr;
   const size_t bytes = sizeof(cx<double>) * (size_t)n * batch;
   HIPCHK(hipMalloc(&d, bytes));
-  hipError_t e = hipMalloc(&t, sizeof(cx<double>) * n);
+  hipError_t e = hipMalloc(&t, sizeof(cx<double>) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(d, inout, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * tw.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_fft_test(n, sign, 0, d, t, batch);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(inout, d, bytes, hipMemcpyDeviceToHost);
@@ -734,9 +737,9 @@ int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int3
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIPCHK(hipMalloc(&a, bytes));
   hipError_t e = hipMalloc(&b, bytes);
-  if (e == hipSuccess) e = hipMalloc(&t, sizeof(cx<double>) * n);
+  if (e == hipSuccess) e = hipMalloc(&t, sizeof(cx<double>) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(a, inout, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * tw.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) e = launch_fft_bench(n, sign, variant & 7, a, b, t, batch, 1);

@@ -40,6 +40,7 @@ struct Geom {
   int nxd, nzd;     // dealiased sizes, physical grid 2nxd x nzd
   int npair;        // Nx*Nz wavenumber pairs, pair p = ix*Nz + (iz+nz)
   size_t ps;        // plane stride (pairs, padded)
+  int ncu;          // compute units of the device (persistent grids)
   double alfa0, beta0;
 };
 

@@ -51,33 +51,79 @@ namespace fft {
 // first pass of radix E (12 or 16, done in registers as 4x3 or 4x4), then
 // radix-4 passes and at most one radix-2 pass.  Otherwise radix-4 passes, at
 // most one radix-2 pass and a final radix-3 pass (fewer live registers, one
-// more LDS exchange).
+// more LDS exchange).  The plan is a set of constexpr functions so that the
+// host builds the twiddle tables from the same definition.
+constexpr bool plan_pow2(int N) { return (N & (N - 1)) == 0; }
+constexpr int plan_E(int N) { return plan_pow2(N) ? 16 : 12; }
+constexpr int plan_rest(int N, bool big) { return big ? N / plan_E(N) : (plan_pow2(N) ? N : N / 3); }
+constexpr int plan_n4(int N, bool big) {
+  int r = plan_rest(N, big), n = 0;
+  while (r % 4 == 0) {
+    r /= 4;
+    n++;
+  }
+  return n;
+}
+constexpr bool plan_has2(int N, bool big) {
+  int r = plan_rest(N, big);
+  while (r % 4 == 0) r /= 4;
+  return r == 2;
+}
+constexpr int plan_npass(int N, bool big) {
+  return (big ? 1 : (plan_pow2(N) ? 0 : 1)) + plan_n4(N, big) + (plan_has2(N, big) ? 1 : 0);
+}
+constexpr int plan_radix(int N, bool big, int p) {
+  if (big) return p == 0 ? plan_E(N) : (p <= plan_n4(N, big) ? 4 : 2);
+  return p < plan_n4(N, big) ? 4 : ((p == plan_n4(N, big) && plan_has2(N, big)) ? 2 : 3);
+}
+
+// Twiddle tables: pass p >= 1 (stride NS = product of the earlier radices,
+// radix RAD) reads w_N^(k*N/(NS*RAD)) for k = j mod NS, stored contiguously
+// per pass so that consecutive lanes read consecutive entries (no LDS bank
+// conflicts when the table sits in LDS).  The tables of the BIG plan come
+// first, those of the other plan follow.
+constexpr int plan_tw_offset(int N, bool big, int P) {
+  int off = 0, ns = 1;
+  for (int q = 0; q < P; q++) {
+    if (q >= 1) off += ns;
+    ns *= plan_radix(N, big, q);
+  }
+  return off;
+}
+constexpr int plan_tw_size(int N, bool big) { return plan_tw_offset(N, big, plan_npass(N, big)); }
+constexpr int tw_table_size(int N) { return plan_tw_size(N, true) + plan_tw_size(N, false); }
+
 template <int N, bool BIG = true>
 struct Plan {
-  static constexpr bool pow2 = (N & (N - 1)) == 0;
-  static constexpr int E = pow2 ? 16 : 12;
+  static constexpr bool pow2 = plan_pow2(N);
+  static constexpr int E = plan_E(N);
   static constexpr int T = N / E;
   static_assert(T >= 1 && T * E == N, "unsupported FFT length");
-  static constexpr int rest() { return BIG ? T : (pow2 ? N : N / 3); }
-  static constexpr int n4() {
-    int r = rest(), n = 0;
-    while (r % 4 == 0) {
-      r /= 4;
-      n++;
-    }
-    return n;
-  }
-  static constexpr bool has2() {
-    int r = rest();
-    while (r % 4 == 0) r /= 4;
-    return r == 2;
-  }
-  static constexpr int npass() { return (BIG ? 1 : (pow2 ? 0 : 1)) + n4() + (has2() ? 1 : 0); }
-  static constexpr int radix(int p) {
-    if (BIG) return p == 0 ? E : (p <= n4() ? 4 : 2);
-    return p < n4() ? 4 : ((p == n4() && has2()) ? 2 : 3);
-  }
+  static constexpr int npass() { return plan_npass(N, BIG); }
+  static constexpr int radix(int p) { return plan_radix(N, BIG, p); }
+  static constexpr int tw_base(int p) { return (BIG ? 0 : plan_tw_size(N, true)) + plan_tw_offset(N, BIG, p); }
 };
+
+// Host: fills w[0 .. tw_table_size(N)) with exp(-2 pi i e/N) in the layout
+// above, from long-double angles.
+template <typename T>
+inline void build_twiddles(cx<T> *w, int N) {
+  int o = 0;
+  for (int big = 1; big >= 0; big--) {
+    int ns = 1;
+    for (int p = 0; p < plan_npass(N, big); p++) {
+      const int rad = plan_radix(N, big, p);
+      if (p >= 1) {
+        const int tws = N / (ns * rad);
+        for (int k = 0; k < ns; k++) {
+          const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)(k * tws) / (long double)N;
+          w[o++] = cx<T>{(T)cosl(ang), (T)sinl(ang)};
+        }
+      }
+      ns *= rad;
+    }
+  }
+}
 
 // team-level LDS synchronisation: a team within one wavefront only needs the
 // wave's LDS operations ordered; larger teams use the workgroup barrier.
@@ -204,14 +250,13 @@ template <int N, int S, int P, int NS, bool BIG, int XCH, typename R>
 __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
   using PL = Plan<N, BIG>;
   constexpr int E = PL::E, T = PL::T, RAD = PL::radix(P), NB = E / RAD;
-  constexpr int TWS = N / (NS * RAD);
 #pragma unroll
   for (int m = 0; m < NB; m++) {
     if constexpr (NS > 1) {
       // one table load per butterfly; higher powers by multiplication
       const int j = t + m * T;
       const int k = j % NS;
-      cx<R> w1 = tw[k * TWS];
+      cx<R> w1 = tw[PL::tw_base(P) + k];
       if constexpr (S > 0) w1.y = -w1.y;
       cx<R> wr = w1;
 #pragma unroll
@@ -269,7 +314,7 @@ __device__ __forceinline__ void run_pass(cx<R> (&v)[Plan<N>::E], R *lds, const c
 
 // Transform the team's N points held in slots e[s] = x[t + T*s]; on return
 // e[s] = X[t + T*s].  lds: lds_reals<N, XCH>() reals private to the team.
-// tw: N twiddles tw[e] = exp(-2 pi i e/N).
+// tw: the twiddle table of build_twiddles (tw_table_size(N) entries).
 template <int N, int S, bool BIG = true, int XCH = 0, typename R>
 __device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const cx<R> *__restrict__ tw, int t) {
   using PL = Plan<N, BIG>;

@@ -147,8 +147,9 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
 
   // twiddles in LDS: their reads then wait on the LDS counter, not behind
   // the row loads still in flight on the vector-memory counter
-  __shared__ __align__(16) cx<R> twl[M];
-  for (int i = tid; i < M; i += NT) twl[i] = tw[i];
+  constexpr int NTW = fft::plan_tw_size(M, XBIG);
+  __shared__ __align__(16) cx<R> twl[NTW > 0 ? NTW : 1];
+  for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
   __syncthreads();
 
   // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
