@@ -154,10 +154,39 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
 
   // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
   // (k <= nx) or the conjugate of mode M-k (k >= M-nx); Im of mode 0 is
-  // dropped as c2r does.  All loads of a component are issued first
-  // (unconditionally, from clamped in-row indices; slots that are zero for
-  // every lane are skipped by a wave-uniform test), the conjugation and
-  // zeroing follow as selects, so the loads overlap one another.
+  // dropped as c2r does.  Teams of at most one wavefront load only the
+  // direct modes (slots s < MO, clamped in-row indices, all loads in flight
+  // together) and take the mirrored ones from the partner lane: mode M-k of
+  // lane t > 0 sits in slot E-1-s of lane T-t, lane 0 holds it in slot E-s.
+  // Wider teams load both.
+  constexpr bool SHFL = T <= 64;
+  auto load_direct = [&](int c, cx<R>(&r)[MO]) {
+    const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB + zi;
+#pragma unroll
+    for (int s = 0; s < MO; s++) r[s] = T * s <= nx ? src[(size_t)min(t + T * s, nx) * ZB] : cx<R>{0, 0};
+  };
+  auto extend = [&](const cx<R>(&r)[MO], cx<R>(&e)[E]) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int src = lane - t + ((T - t) % T);
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int k = t + T * s;
+      cx<R> val = {0, 0};
+      if (s < MO) {
+        val = r[s < MO ? s : 0];
+        if (k > nx) val = cx<R>{0, 0};
+        if (k == 0) val.y = 0;
+      }
+      if (s >= E - MO) {
+        const int sm = E - 1 - s;
+        const R px = __shfl(r[sm].x, src, 64), py = __shfl(r[sm].y, src, 64);
+        cx<R> mv = cx<R>{px, py};
+        if (t == 0) mv = (E - s < MO) ? r[(E - s < MO) ? E - s : 0] : cx<R>{0, 0};
+        if (k >= M - nx) val = cx<R>{mv.x, -mv.y};
+      }
+      e[s] = val;
+    }
+  };
   auto load_raw = [&](int c, cx<R>(&e)[E]) {
     const cx<R> *src = A + ((size_t)(pl * 3 + c) * NZB + zb) * Nx * ZB + zi;
 #pragma unroll
@@ -190,20 +219,33 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   R u[E], v[E], w[E];
   {
     cx<R> e[E], f[E];
-    load_raw(0, e);
-    load_raw(1, f);
-    fix_ext(e);
-    fix_ext(f);
+    cx<R> rw[SHFL ? MO : 1];
+    if constexpr (SHFL) {
+      cx<R> ru[MO], rv[MO];
+      load_direct(0, ru);
+      load_direct(1, rv);
+      load_direct(2, rw);  // in flight during the first transform
+      extend(ru, e);
+      extend(rv, f);
+    } else {
+      load_raw(0, e);
+      load_raw(1, f);
+      fix_ext(e);
+      fix_ext(f);
+    }
 #pragma unroll
     for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
-    load_raw(2, f);  // in flight during the first transform
+    if constexpr (!SHFL) load_raw(2, f);  // in flight during the first transform
     fft::fft_slots<M, +1, XBIG, XXCH>(e, fr, twl, t);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       u[s] = e[s].x;
       v[s] = e[s].y;
     }
-    fix_ext(f);
+    if constexpr (SHFL)
+      extend(rw, f);
+    else
+      fix_ext(f);
     fft::fft_slots<M, +1, XBIG, XXCH>(f, fr, twl, t);
 #pragma unroll
     for (int s = 0; s < E; s++) w[s] = f[s].x;
