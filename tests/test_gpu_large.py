@@ -1,0 +1,105 @@
+"""Parity at the production sizes.
+
+191x384x189 is too large for the oracle to follow in seconds, so the HIP
+step is checked there through properties that hold at any size:
+- continuity i kx u + v' + i kz w = 0, with v' the compact derivative of v
+  (COMPLEXderiv, dnsdata.f90:337-373) restated here as a banded solve with
+  LAPACK (scipy.linalg.solve_banded) from the host set-up;
+- Hermitian symmetry of the kx = 0 plane;
+- v = 0 in the (0,0) mode;
+- a finite field and Runtimedata.
+95x128x95 is compared with the oracle over 10 steps at the FP64 bar."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from cases import config2, config3, rel_l2, seeded_field
+from channel_amd import lib
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+orc.set_threads(min(16, os.cpu_count() or 1))
+
+
+def compact_derivative(cfg, F):
+    """dF/dy on rows -1..ny+1 for F of shape (ny+3, m): the one-sided wall
+    and ghost rows, then D0 F' = D1 F on rows 1..ny-1 with the wall/ghost
+    values moved to the right-hand side, solved as one banded system."""
+    s = lib.setup_host(cfg)
+    der, bnd = s["der"], s["bnd"]  # der[iy-1] = (d0, d1, d2, d4); bnd rows d040 d140 d240 d14m1 d24m1 d04n d14n d24n d14np1 d24np1
+    ny = cfg.ny
+    d140, d14m1, d14n, d14np1 = bnd[1], bnd[3], bnd[6], bnd[8]
+    out = np.zeros_like(F)
+    out[1] = d140 @ F[0:5]                # row 0 (index iy+1)
+    out[0] = d14m1 @ F[0:5]               # row -1
+    out[ny + 1] = d14n @ F[ny - 2:ny + 3]  # row ny
+    out[ny + 2] = d14np1 @ F[ny - 2:ny + 3]
+    n = ny - 1
+    ab = np.zeros((5, n))
+    rhs = np.zeros((n, F.shape[1]), dtype=F.dtype)
+    for i in range(n):
+        iy = i + 1
+        d0, d1 = der[iy - 1][0], der[iy - 1][1]
+        rhs[i] = d1 @ F[iy - 1:iy + 4]    # rows iy-2..iy+2
+        for j in range(5):
+            r = iy - 2 + j                # row of the unknown F'
+            if 1 <= r <= ny - 1:
+                ab[2 + i - (r - 1), r - 1] = d0[j]
+            else:
+                rhs[i] -= d0[j] * out[r + 1]
+    out[2:ny + 1] = scipy.linalg.solve_banded((2, 2), ab, rhs)
+    return out
+
+
+def test_config3_properties():
+    from channel_amd.solver import Channel
+    cfg = config3()
+    print("[case] config3 2 steps, properties", flush=True)
+    g = Channel(cfg)
+    g.set_field(seeded_field(cfg))
+    g.initial()
+    rows = [g.outstats()] + [g.step() for _ in range(2)]
+    assert np.all(np.isfinite(rows))
+    V = g.get_field()
+    g.close()
+    assert np.all(np.isfinite(V))
+    nx, nz, ny = cfg.nx, cfg.nz, cfg.ny
+    # Hermitian symmetry of kx = 0
+    P0 = V[:, :, 0, :]
+    err = np.max(np.abs(P0[:, ::-1, :] - np.conj(P0)))
+    assert err <= 1e-13 * np.max(np.abs(P0)), err
+    # (0,0) mode: no mean wall-normal velocity
+    assert np.max(np.abs(V[:, nz, 0, 1])) == 0.0
+    # continuity on every pair but (0,0)
+    v = V[:, :, :, 1].reshape(ny + 3, -1)
+    dv = compact_derivative(cfg, v).reshape(ny + 3, 2 * nz + 1, nx + 1)
+    kx = cfg.alfa0 * np.arange(nx + 1)[None, None, :]
+    kz = cfg.beta0 * np.arange(-nz, nz + 1)[None, :, None]
+    u, w = V[:, :, :, 0], V[:, :, :, 2]
+    res = 1j * kx * u + dv + 1j * kz * w
+    scale = np.abs(kx * u) + np.abs(dv) + np.abs(kz * w)
+    res[:, nz, 0] = 0.0
+    scale[:, nz, 0] = 1.0
+    rel = np.max(np.abs(res)) / np.max(scale)
+    print("continuity residual / scale:", rel)
+    assert rel <= 1e-12
+
+
+def test_config2_10_steps():
+    from channel_amd.solver import Channel
+    cfg = config2()
+    print("[case] config2 10 steps vs oracle", flush=True)
+    V = seeded_field(cfg)
+    g = Channel(cfg)
+    g.set_field(V)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    for k in range(10):
+        rg, ro = g.step(), o.step()
+        scale = np.maximum(np.abs(ro), 1.0)
+        np.testing.assert_array_less(np.abs(rg - ro), 1e-11 * scale)
+    e = rel_l2(g.get_field(), o.get_field())
+    print("config2 rel L2 after 10 steps:", e)
+    assert e <= 1e-12
