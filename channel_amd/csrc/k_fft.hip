@@ -22,12 +22,34 @@ namespace chan {
 // their LDS footprint small
 constexpr int ZXCH = fft::XCH_PAD;
 
+// FP32 z blocks are 32 bytes per x mode, too short for efficient memory
+// transactions: in FP32 the z kernels stage through LDS, where the G x
+// modes of a block form one contiguous run of G*ZB values per z block, and
+// lanes move whole runs (a wave touches 4 runs of 128 bytes).
+template <int NZ, typename R>
+constexpr bool z_staged() {
+  return sizeof(R) == 4;
+}
+// team stride of the staging area (a 32-byte shift per team keeps the
+// staged writes and reads free of LDS bank conflicts)
+template <int NZ>
+constexpr int stage_stride() {
+  return NZ + 4;
+}
+template <int NZ, typename R>
+constexpr int zbuf_reals() {
+  constexpr int G = 256 / fft::Plan<NZ>::T;
+  constexpr int fftr = G * fft::lds_reals<NZ, ZXCH>();
+  constexpr int stg = z_staged<NZ, R>() ? 2 * G * stage_stride<NZ>() : 0;
+  return fftr > stg ? fftr : stg;
+}
+
 template <int NZ, typename R>
 __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, cx<R> *__restrict__ A,
                                               const cx<R> *__restrict__ tw, Geom g, int plane0) {
   using PL = fft::Plan<NZ>;
-  constexpr int T = PL::T, E = PL::E, G = 256 / T;
-  __shared__ __align__(16) R lds[G * fft::lds_reals<NZ, ZXCH>()];
+  constexpr int T = PL::T, E = PL::E, G = 256 / T, SP = stage_stride<NZ>();
+  __shared__ __align__(16) R lds[zbuf_reals<NZ, R>()];
   const int team = threadIdx.x / T, t = threadIdx.x % T;
   const int nxg = (g.Nx + G - 1) / G;
   int b = blockIdx.x;
@@ -47,8 +69,21 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
     e[s] = cx<R>{(R)v.x, (R)v.y};
   }
   fft::fft_slots<NZ, +1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
-  if (act) {
-    cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
+  cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
+  if constexpr (z_staged<NZ, R>()) {
+    cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
+    __syncthreads();  // every team is done with its transform exchanges
+#pragma unroll
+    for (int s = 0; s < E; s++) stg[team * SP + t + T * s] = e[s];
+    __syncthreads();
+    const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int el = (int)threadIdx.x + 256 * i;  // G*NZ = 256*E elements per block
+      const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
+      if (tm < nact) dst[((size_t)zb * g.Nx + ix0) * ZB + w] = stg[tm * SP + zb * ZB + r];
+    }
+  } else if (act) {
 #pragma unroll
     for (int s = 0; s < E; s++) {
       const int z = t + T * s;
@@ -310,8 +345,8 @@ template <int NZ, typename R>
 __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R> *__restrict__ P,
                                               const cx<R> *__restrict__ tw, Geom g, int plane0) {
   using PL = fft::Plan<NZ>;
-  constexpr int T = PL::T, E = PL::E, G = 256 / T;
-  __shared__ __align__(16) R lds[G * fft::lds_reals<NZ, ZXCH>()];
+  constexpr int T = PL::T, E = PL::E, G = 256 / T, SP = stage_stride<NZ>();
+  __shared__ __align__(16) R lds[zbuf_reals<NZ, R>()];
   const int team = threadIdx.x / T, t = threadIdx.x % T;
   const int nxg = (g.Nx + G - 1) / G;
   int b = blockIdx.x;
@@ -322,10 +357,32 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
   const bool act = ix < g.Nx;
   const cx<R> *src = B + (size_t)(pl * 6 + q) * NZ * g.Nx;
   cx<R> e[E];
+  if constexpr (z_staged<NZ, R>()) {
+    cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
+    const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
+    cx<R> v[E];
 #pragma unroll
-  for (int s = 0; s < E; s++) {
-    const int z = t + T * s;
-    e[s] = act ? src[((size_t)(z / ZB) * g.Nx + ix) * ZB + (z % ZB)] : cx<R>{0, 0};
+    for (int i = 0; i < E; i++) {  // all loads in flight together, clamped into the block's runs
+      const int el = (int)threadIdx.x + 256 * i;
+      const int zb = el / (G * ZB), w = el % (G * ZB);
+      v[i] = src[((size_t)zb * g.Nx + ix0) * ZB + min(w, nact * ZB - 1)];
+    }
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int el = (int)threadIdx.x + 256 * i;
+      const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
+      stg[tm * SP + zb * ZB + r] = tm < nact ? v[i] : cx<R>{0, 0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; s++) e[s] = stg[team * SP + t + T * s];
+    __syncthreads();  // the transforms reuse the staging area
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; s++) {
+      const int z = t + T * s;
+      e[s] = act ? src[((size_t)(z / ZB) * g.Nx + ix) * ZB + (z % ZB)] : cx<R>{0, 0};
+    }
   }
   fft::fft_slots<NZ, -1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
   if (act) {
