@@ -28,7 +28,7 @@ constexpr int ZXCH = fft::XCH_PAD;
 // lanes move whole runs (a wave touches 4 runs of 128 bytes).
 template <int NZ, typename R>
 constexpr bool z_staged() {
-  return sizeof(R) == 4;
+  return true;
 }
 // team stride of the staging area (a 32-byte shift per team keeps the
 // staged writes and reads free of LDS bank conflicts)
