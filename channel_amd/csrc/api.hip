@@ -122,7 +122,8 @@ struct chan_ctx {
   Geom g;
   hipStream_t st = nullptr;
   // device state
-  cx<double> *V = nullptr, *old = nullptr, *rhs = nullptr, *col00 = nullptr;
+  cx<double> *V = nullptr, *rhs = nullptr, *col00 = nullptr;
+  void *old = nullptr;  // explicit terms: complex FP64, or FP32 in the split mode
   double2 *L = nullptr;
   // transform intermediates and twiddles: complex FP64, or complex FP32 in
   // the split mode
@@ -287,11 +288,11 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   const size_t plane = g.ps, NyP = (size_t)g.Ny * plane;
   const size_t slab = (size_t)g.nzd * g.Nx;  // one component of one plane in physical z
   if ((rc = dalloc(&c->V, 3 * NyP))) return bail(rc);
-  if ((rc = dalloc(&c->old, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->rhs, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->L, 2 * (size_t)(g.ny - 1) * plane))) return bail(rc);
   if ((rc = dalloc(&c->col00, 3 * (size_t)g.Ny + STASH))) return bail(rc);
   const size_t esz = c->split ? sizeof(cx<float>) : sizeof(cx<double>);
+  if ((rc = dalloc_bytes(&c->old, 2 * (size_t)(g.ny - 1) * plane * esz))) return bail(rc);
   // B doubles as the staging area of the field and rhs transposes, which
   // needs 3*Ny*npair complex FP64 <= 6*Ny*slab*esz
   if ((rc = dalloc_bytes(&c->A, 3 * (size_t)g.Ny * slab * esz))) return bail(rc);
@@ -326,7 +327,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   s0.meanpz = p->meanpz;
   if (hipMemcpy(c->sc, &s0, sizeof s0, hipMemcpyHostToDevice) != hipSuccess) return bail(fail(CHAN_E_STATE, "upload failed"));
   // stored explicit terms start at zero (ODE3 = 0 in the first substep)
-  if (hipMemset(c->old, 0, sizeof(cx<double>) * 2 * (g.ny - 1) * plane) != hipSuccess ||
+  if (hipMemset(c->old, 0, esz * 2 * (g.ny - 1) * plane) != hipSuccess ||
       hipMemset(c->V, 0, sizeof(cx<double>) * 3 * NyP) != hipSuccess)
     return bail(fail(CHAN_E_STATE, "hipMemset failed"));
   *out = c;
@@ -500,7 +501,7 @@ static int rhs_t(chan_ctx *c, const double ode[3]) {
   RhsArgs<R> a;
   a.P = static_cast<const cx<R> *>(c->P);
   a.V = c->V;
-  a.old = c->old;
+  a.old = static_cast<cx<R> *>(c->old);
   a.rhs = c->rhs;
   a.der = c->der;
   a.sc = c->sc;

@@ -68,7 +68,8 @@ template <typename R>
 struct RhsArgs {
   const cx<R> *P;        // [6][Ny][ps] products
   const cx<double> *V;   // [3][Ny][ps]
-  cx<double> *old;       // [2][ny-1][ps] previous explicit terms (eta, d2v)
+  cx<R> *old;            // [2][ny-1][ps] previous explicit terms (eta, d2v); in the split
+                         // mode FP32, as the products they are formed from
   cx<double> *rhs;       // [2][ny-1][ps]
   const double *der;     // [(ny-1)][4][5]
   const DevScalars *sc;  // meanpx, meanpz

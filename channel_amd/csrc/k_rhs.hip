@@ -50,16 +50,20 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
   };
 #pragma unroll
   for (int j = 0; j < 4; j++) load_row(y0 - 2 + j, j);
-  nxo[0] = a.old[(size_t)(y0 - 1) * ps + p];
-  nxo[1] = a.old[rs + (size_t)(y0 - 1) * ps + p];
+  auto ldold = [&](size_t i) {
+    const cx<R> o = a.old[i];
+    return cx<double>{(double)o.x, (double)o.y};
+  };
+  nxo[0] = ldold((size_t)(y0 - 1) * ps + p);
+  nxo[1] = ldold(rs + (size_t)(y0 - 1) * ps + p);
 
   for (int iy = y0; iy <= y1; iy++) {
     load_row(iy + 2, 4);
     const cx<double> old_eta = nxo[0], old_v = nxo[1];
     {
       const int on = min(iy, g.ny - 2);
-      nxo[0] = a.old[(size_t)on * ps + p];
-      nxo[1] = a.old[rs + (size_t)on * ps + p];
+      nxo[0] = ldold((size_t)on * ps + p);
+      nxo[1] = ldold(rs + (size_t)on * ps + p);
     }
     const double *d = der + (size_t)(iy - 1) * 20;  // d0[5] d1[5] d2[5] d4[5]
     auto DD = [&](int f, int q) {
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       const cx<double> old = old_v;
       const cx<double> un = scl(a.ode0, unkn);
       const cx<double> r = sub(add(add(cx<double>{un.x / a.dt, un.y / a.dt}, impl), scl(a.ode1, expl)), scl(a.ode2, old));
-      a.old[rs + o] = expl;
+      a.old[rs + o] = cx<R>{(R)expl.x, (R)expl.y};
       a.rhs[rs + o] = r;
     }
     // eta equation (or the mean flow Re u + i Re w in mode (0,0))
@@ -133,7 +137,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       const cx<double> old = old_eta;
       const cx<double> un = scl(a.ode0, unkn);
       const cx<double> r = sub(add(add(cx<double>{un.x / a.dt, un.y / a.dt}, impl), scl(a.ode1, ex)), scl(a.ode2, old));
-      a.old[o] = ex;
+      a.old[o] = cx<R>{(R)ex.x, (R)ex.y};
       a.rhs[o] = r;
     }
 #pragma unroll
