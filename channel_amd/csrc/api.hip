@@ -478,7 +478,12 @@ template <typename R>
 static int products_t(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
   const Geom &g = c->g;
   const HostSetup &h = c->hs;
-  cx<R> *A = static_cast<cx<R> *>(c->A), *B = static_cast<cx<R> *>(c->B), *P = static_cast<cx<R> *>(c->P);
+  // the transform kernels index A and B from the launch's first plane: the
+  // base pointers are offset so that every plane keeps its own slices
+  const size_t slab = (size_t)g.nzd * g.Nx;
+  cx<R> *A = static_cast<cx<R> *>(c->A) + (size_t)plane0 * 3 * slab;
+  cx<R> *B = static_cast<cx<R> *>(c->B) + (size_t)plane0 * 6 * slab;
+  cx<R> *P = static_cast<cx<R> *>(c->P);
   const cx<R> *twz = static_cast<const cx<R> *>(c->twz), *twx = static_cast<const cx<R> *>(c->twx);
   int rc;
   if ((rc = timed(c, "zinv", [&] { return launch_zinv<R>(g.nzd, c->st, c->V, A, twz, g, plane0, nplanes); }))) return rc;
