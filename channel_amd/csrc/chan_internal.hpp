@@ -108,6 +108,9 @@ struct UwArgs {
   Geom g;
 };
 
+// records the message of a failing C-ABI call (chan_last_error) and returns code
+int set_error(int code, const char *fmt, ...);
+
 // ----------------------------------------------------------- launchers --
 int fft_size_supported(int N);
 template <typename R>

@@ -46,6 +46,7 @@ class ChanScalars(ctypes.Structure):
 # every symbol the header declares, with (restype, argtypes)
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.c_int32
 SIGNATURES = {
     "chan_abi_version": (ctypes.c_int, []),
@@ -74,6 +75,16 @@ SIGNATURES = {
     "chan_fft_selftest": (ctypes.c_int, [_I32, _I32, _I32, _D]),
     "chan_copy_selftest": (ctypes.c_int, [ctypes.c_int64, _I32, _D]),
     "chan_fft_bench": (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32, _D, _D]),
+    # include/channel_closure.h
+    "chan_mlp_create": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
+    "chan_mlp_destroy": (ctypes.c_int, [_P]),
+    "chan_mlp_set_params": (ctypes.c_int, [_P, _F]),
+    "chan_mlp_get_params": (ctypes.c_int, [_P, _F]),
+    "chan_mlp_set_data": (ctypes.c_int, [_P, _F, _F, ctypes.c_int64]),
+    "chan_mlp_forward": (ctypes.c_int, [_P, _F, ctypes.c_int64, _F]),
+    "chan_mlp_grad": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _F, _D]),
+    "chan_mlp_train": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _F, _D]),
+    "chan_mlp_bench_forward": (ctypes.c_int, [_P, _I32, _D]),
 }
 
 
@@ -116,6 +127,10 @@ def check(rc):
 
 def dptr(a):
     return a.ctypes.data_as(_D)
+
+
+def fptr(a):
+    return a.ctypes.data_as(_F)
 
 
 def params_from(cfg, precision=CHAN_FP64):

@@ -1,0 +1,68 @@
+/*
+ * channel_closure.h -- C-ABI of the learned subgrid-stress closure that
+ * rides on the MI355X channel step: a 46 -> 64 -> 64 -> 64 -> 6 ReLU MLP with
+ * BF16 weights and activations and FP32 accumulation on the matrix cores.
+ *
+ * The reference has no counterpart (it is a DNS code without subgrid
+ * model).  Inputs are box-filtered velocity-gradient features, outputs the
+ * six subgrid stresses tau_xx, tau_yy, tau_zz, tau_xy, tau_xz, tau_yz (the
+ * feature and target definitions live with the dataset builder,
+ * channel_amd/closure.py).
+ *
+ * Conventions as in channel_hip.h: int status (0 ok, > 0 HIP error, < 0
+ * CHAN_E_*), chan_last_error() for the message, plain host pointers, one host
+ * thread per handle.
+ *
+ * Parameters travel in the canonical FP32 layout W1[46][64], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], W4[64][6], b4[6]
+ * (CHAN_MLP_NPARAM floats; W[in][out]).  The library keeps FP32 master
+ * weights and BF16 copies for the kernels.
+ */
+#ifndef CHANNEL_CLOSURE_H
+#define CHANNEL_CLOSURE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHAN_MLP_IN 46
+#define CHAN_MLP_OUT 6
+#define CHAN_MLP_NPARAM 11718
+
+typedef struct chan_mlp chan_mlp;
+
+/* Allocates device storage for up to `capacity` resident samples. */
+int chan_mlp_create(int64_t capacity, chan_mlp **out);
+int chan_mlp_destroy(chan_mlp *m);
+
+int chan_mlp_set_params(chan_mlp *m, const float *params);
+int chan_mlp_get_params(chan_mlp *m, float *params);
+
+/* Uploads n <= capacity samples: features X[n][46] (rounded to BF16 on the
+ * device) and targets T[n][6] (FP32).  Resets the Adam moments. */
+int chan_mlp_set_data(chan_mlp *m, const float *X, const float *T, int64_t n);
+
+/* Inference on host rows X[n][46] -> Y[n][6]; n <= capacity. */
+int chan_mlp_forward(chan_mlp *m, const float *X, int64_t n, float *Y);
+
+/* Loss sum((y - t)^2) / (6 count) over resident samples [first, first+count)
+ * and its gradient in the canonical layout (FP32 accumulation of BF16
+ * operands; the stored activations and dz are BF16). */
+int chan_mlp_grad(chan_mlp *m, int64_t first, int64_t count, float *grad, double *loss);
+
+/* `steps` Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) over consecutive
+ * minibatches of `batch` resident samples, wrapping around; the loss of each
+ * step before its update goes to losses[steps] (may be NULL).  Returns the
+ * average duration of one step in *ms_per_step (may be NULL). */
+int chan_mlp_train(chan_mlp *m, int32_t steps, int32_t batch, float lr, float *losses, double *ms_per_step);
+
+/* Timed inference over the resident samples: `reps` forward passes, average
+ * milliseconds per pass in *ms. */
+int chan_mlp_bench_forward(chan_mlp *m, int32_t reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

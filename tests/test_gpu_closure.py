@@ -1,0 +1,93 @@
+"""The closure MLP kernels (BF16 MFMA, FP32 accumulation) against the CPU
+float64 checker that rounds to BF16 where the device stores BF16.  The
+remaining differences come from the FP32 accumulation order, which can move
+a stored value across a BF16 rounding boundary now and then: hence relative
+L2 bounds of 1e-4 (forward) and 1e-3 (gradient).  Training is checked on a
+teacher network (the loss must fall) and on one exact Adam step."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def setup(n, seed=0):
+    from channel_amd.closure import Mlp
+    rng = np.random.default_rng(seed)
+    p = (mo.init_params(seed) + 0.05 * rng.standard_normal(mo.NPARAM)).astype(np.float32).astype(np.float64)
+    X = rng.standard_normal((n, mo.IN)).astype(np.float32)
+    T = rng.standard_normal((n, mo.OUT)).astype(np.float32)
+    m = Mlp(n)
+    m.set_params(p)
+    m.set_data(X, T)
+    return m, p, X.astype(np.float64), T.astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [1, 37, 1000, 20000])
+def test_forward_matches_checker(n):
+    print("[case] mlp forward", n, flush=True)
+    m, p, X, T = setup(n)
+    y = m.forward(X)
+    ref = mo.mirrored(p, X)[0]
+    e = rel_l2(y, ref)
+    print("forward rel L2", e)
+    assert e < 1e-4
+    np.testing.assert_allclose(m.get_params(), p.astype(np.float32))
+    m.close()
+
+
+@pytest.mark.parametrize("n", [16, 300, 5000])
+def test_gradient_matches_checker(n):
+    print("[case] mlp gradient", n, flush=True)
+    m, p, X, T = setup(n, seed=n)
+    loss, g = m.grad(0, n)
+    _, lref, gref = mo.mirrored(p, X, T)
+    print("loss", loss, lref, "grad rel L2", rel_l2(g, gref))
+    assert abs(loss - lref) <= 1e-4 * lref
+    assert rel_l2(g, gref) < 1e-3
+    # the exact float64 gradient differs only by the BF16 roundings
+    gex = mo.exact(p, X, T)[2]
+    assert rel_l2(g, gex) < 5e-2
+    # a sub-range of the resident samples
+    loss2, g2 = m.grad(n // 4, n // 2)
+    _, l2ref, g2ref = mo.mirrored(p, X[n // 4:n // 4 + n // 2], T[n // 4:n // 4 + n // 2])
+    assert rel_l2(g2, g2ref) < 1e-3
+    m.close()
+
+
+def test_one_adam_step():
+    print("[case] mlp adam step", flush=True)
+    n = 2048
+    m, p, X, T = setup(n, seed=9)
+    _, g = m.grad(0, n)
+    lr = 1e-3
+    m.train(1, n, lr)
+    p1 = m.get_params().astype(np.float64)
+    # first Adam step: m_hat = g, v_hat = g^2, update lr * g / (|g| + eps)
+    expect = p.astype(np.float32).astype(np.float64) - lr * g / (np.abs(g) + 1e-8)
+    big = np.abs(g) > 1e-5
+    np.testing.assert_allclose(p1[big], expect[big], rtol=0, atol=1e-6)
+    m.close()
+
+
+def test_training_fits_a_teacher():
+    print("[case] mlp teacher-student", flush=True)
+    from channel_amd.closure import Mlp
+    rng = np.random.default_rng(42)
+    n = 65536
+    X = rng.standard_normal((n, mo.IN))
+    teacher = mo.init_params(7)
+    T = mo.exact(teacher, X)[0]
+    m = Mlp(n)
+    m.set_params(mo.init_params(11).astype(np.float32))
+    m.set_data(X, T)
+    losses, ms = m.train(400, 4096, 2e-3)
+    print("loss %.4g -> %.4g, %.3f ms/step" % (losses[0], losses[-10:].mean(), ms))
+    assert np.all(np.isfinite(losses))
+    assert losses[-10:].mean() < 0.1 * losses[0]
+    m.close()
