@@ -147,3 +147,78 @@ def dataset(cfg, V, ratio=4):
     X = X.reshape(46, -1).T
     T = T.reshape(6, -1).T
     return X.astype(np.float32), T.astype(np.float32)
+
+
+# ------------------------------------------------------- a priori study --
+def _smagorinsky_basis(X):
+    """-|S| S_ij for the six stress components from the gradient features at
+    the point (the eddy-viscosity form; (Cs Delta)^2 is fitted)."""
+    g = X[:, :9].astype(np.float64).reshape(-1, 3, 3)  # g[:, i, j] = du_i/dx_j
+    S = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    mag = np.sqrt(2.0 * np.sum(S * S, axis=(1, 2)))
+    idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    B = np.stack([-mag * S[:, i, j] for i, j in idx], axis=1)
+    tr = S[:, 0, 0] + S[:, 1, 1] + S[:, 2, 2]
+    B[:, :3] += (mag * tr / 3.0)[:, None]  # deviatoric
+    return B
+
+
+def _corr(a, b):
+    a = a - a.mean()
+    b = b - b.mean()
+    return float(np.sum(a * b) / np.sqrt(np.sum(a * a) * np.sum(b * b)))
+
+
+def _he_init(seed):
+    rng = np.random.default_rng(seed)
+    shapes = [(IN, 64), (64,), (64, 64), (64,), (64, 64), (64,), (64, OUT), (OUT,)]
+    parts = []
+    for s in shapes:
+        if len(s) == 2:
+            lim = np.sqrt(6.0 / s[0])
+            parts.append(rng.uniform(-lim, lim, size=s).ravel())
+        else:
+            parts.append(np.zeros(s))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def a_priori(cfg, V0, steps_between=40, nsnap=3, ratio=4, train_steps=1500, batch=8192, lr=2e-3, seed=3,
+             log=print):
+    """Runs the DNS from V0 on the GPU, builds filtered datasets from nsnap
+    snapshots (training on the first nsnap-1, testing on the last), trains
+    the MLP on standardised features/targets and returns held-out
+    correlations per stress component for the MLP and for a least-squares
+    Smagorinsky fit."""
+    from .solver import Channel
+    ch = Channel(cfg)
+    ch.set_field(V0)
+    if cfg.cflmax > 0:  # adaptive time step: the pre-loop CFL pass sets deltat
+        ch.initial()
+        ch.outstats()
+    snaps = []
+    for _ in range(nsnap):
+        ch.steps(steps_between)
+        snaps.append(ch.get_field())
+    ch.close()
+    data = [dataset(cfg, V, ratio) for V in snaps]
+    Xtr = np.concatenate([d[0] for d in data[:-1]])
+    Ttr = np.concatenate([d[1] for d in data[:-1]])
+    Xte, Tte = data[-1]
+    xm, xs = Xtr.mean(0), Xtr.std(0) + 1e-12
+    tm, ts = Ttr.mean(0), Ttr.std(0) + 1e-30
+    m = Mlp(len(Xtr))
+    m.set_params(_he_init(seed))
+    m.set_data((Xtr - xm) / xs, (Ttr - tm) / ts)
+    losses, ms = m.train(train_steps, batch, lr)
+    Yte = m.forward((Xte - xm) / xs) * ts + tm
+    m.close()
+    corr_mlp = [_corr(Yte[:, c], Tte[:, c]) for c in range(OUT)]
+    # Smagorinsky: tau_ij = C B_ij with one C for the deviatoric part,
+    # least squares on the training set
+    Btr, Bte = _smagorinsky_basis(Xtr), _smagorinsky_basis(Xte)
+    C = float(np.sum(Btr * (Ttr - Ttr.mean(0))) / np.sum(Btr * Btr))
+    corr_smag = [_corr(C * Bte[:, c], Tte[:, c]) for c in range(OUT)]
+    out = dict(n_train=len(Xtr), n_test=len(Xte), loss_first=float(losses[0]), loss_last=float(losses[-20:].mean()),
+               ms_per_step=ms, corr_mlp=corr_mlp, corr_smagorinsky=corr_smag)
+    log(out)
+    return out
