@@ -144,6 +144,10 @@ struct chan_ctx {
   void *A = nullptr, *B = nullptr, *P = nullptr, *twz = nullptr, *twx = nullptr;
   double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr;
   double *stat = nullptr;  // statistics: [3][ny+1] mean in, [7][ny+1] profiles out
+  // body force: static field [3][Ny][ps] and feedback gains [3][Ny][Nz+1]
+  cx<double> *F0 = nullptr;
+  double *gain = nullptr;
+  int src[3] = {-1, -1, -1};
   DevScalars *sc = nullptr;
   // host scalars
   double time = 0, deltat = 0;
@@ -356,7 +360,7 @@ int chan_destroy(chan_ctx *c) {
   }
   for (auto e : c->pool) hipEventDestroy(e);
   void *bufs[] = {c->V, c->old, c->rhs, c->L, c->col00, c->A, c->B, c->P, c->twz, c->twx,
-                  c->der, c->d0fac, c->y, c->dy, c->sc, c->stat};
+                  c->der, c->d0fac, c->y, c->dy, c->sc, c->stat, c->F0, c->gain};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (c->st) hipStreamDestroy(c->st);
@@ -422,6 +426,74 @@ int chan_get_field(chan_ctx *c, double *V) {
   cx<double> *stage = static_cast<cx<double> *>(c->B);
   HIPCHK(launch_to_host_layout(c->st, c->V, stage, c->g));
   HIPCHK(hipMemcpyAsync(V, stage, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int chan_set_body_force(chan_ctx *c, const double *F) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (!F) {
+    if (c->F0) HIPCHK(hipFree(c->F0));
+    c->F0 = nullptr;
+    return 0;
+  }
+  int rc;
+  if (!c->F0 && (rc = dalloc(&c->F0, 3 * (size_t)c->g.Ny * c->g.ps))) return rc;
+  const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
+  HIPCHK(hipMemcpyAsync(stage, F, n * sizeof(cx<double>), hipMemcpyHostToDevice, c->st));
+  HIPCHK(launch_to_device_layout(c->st, stage, c->F0, c->g));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int chan_set_body_force_feedback(chan_ctx *c, const int32_t src[3], const double *gain) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  bool on = src && gain;
+  if (on) {
+    on = false;
+    for (int k = 0; k < 3; k++) {
+      if (src[k] < -1 || src[k] > 2) return fail(CHAN_E_ARG, "feedback source component %d of force component %d is not -1, 0, 1 or 2", src[k], k);
+      on = on || src[k] >= 0;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (!on) {
+    if (c->gain) HIPCHK(hipFree(c->gain));
+    c->gain = nullptr;
+    return 0;
+  }
+  const size_t n = (size_t)3 * c->g.Ny * (c->g.Nz + 1);
+  int rc;
+  if (!c->gain && (rc = dalloc(&c->gain, n))) return rc;
+  HIPCHK(hipMemcpy(c->gain, gain, n * sizeof(double), hipMemcpyHostToDevice));
+  for (int k = 0; k < 3; k++) c->src[k] = src[k];
+  return 0;
+}
+
+int chan_get_body_force(chan_ctx *c, double *F) {
+  if (!c || !F) return fail(CHAN_E_ARG, "null argument");
+  const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
+  if (!c->F0 && !c->gain) {
+    std::memset(F, 0, n * sizeof(cx<double>));
+    return 0;
+  }
+  // the products buffer (dead between calls; 6*Ny*ps elements of at least
+  // 8 bytes) holds the force in the device layout, 3*Ny*ps complex FP64
+  cx<double> *dev = static_cast<cx<double> *>(c->P);
+  ForceArgs a;
+  a.V = c->V;
+  a.F0 = c->F0;
+  a.gain = c->gain;
+  a.der = c->der;
+  for (int k = 0; k < 3; k++) a.src[k] = c->gain ? c->src[k] : -1;
+  a.out = dev;
+  a.g = c->g;
+  HIPCHK(launch_force(c->st, a));
+  HIPCHK(launch_to_host_layout(c->st, dev, stage, c->g));
+  HIPCHK(hipMemcpyAsync(F, stage, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -530,6 +602,10 @@ static int rhs_t(chan_ctx *c, const double ode[3]) {
   a.dt = c->deltat;
   a.ni = c->prm.ni;
   a.chunk = 48;
+  a.F0 = c->F0;
+  a.gain = c->gain;
+  for (int k = 0; k < 3; k++) a.src[k] = c->gain ? c->src[k] : -1;
+  a.force = c->F0 || c->gain;
   return timed(c, "rhs", [&] { return launch_rhs<R>(c->st, a); });
 }
 }  // extern "C++"

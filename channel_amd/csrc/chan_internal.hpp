@@ -73,9 +73,23 @@ struct RhsArgs {
   cx<double> *rhs;       // [2][ny-1][ps]
   const double *der;     // [(ny-1)][4][5]
   const DevScalars *sc;  // meanpx, meanpz
+  // body force F_d = F0_d + gain[d][plane][col] * V_src[d] (either part may be
+  // absent: F0 null, src[d] < 0); col = iz+nz, or Nz for the mode (0,0)
+  const cx<double> *F0;  // [3][Ny][ps] or null
+  const double *gain;    // [3][Ny][Nz+1] or null
+  int src[3];
+  bool force;
   Geom g;
   double ode0, ode1, ode2, dt, ni;
   int chunk;
+};
+
+struct ForceArgs {
+  const cx<double> *V, *F0;
+  const double *gain, *der;
+  int src[3];
+  cx<double> *out;  // [3][Ny][ps]
+  Geom g;
 };
 
 struct SolveArgs {
@@ -128,6 +142,7 @@ hipError_t launch_fft_bench(int N, int sign, int variant, const cx<double> *in, 
                             int batch, int inner);
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a);
+hipError_t launch_force(hipStream_t st, const ForceArgs &a);
 hipError_t launch_stats(hipStream_t st, const cx<double> *V, const double *avg, double *out, const Geom &g);
 hipError_t launch_solve(hipStream_t st, const SolveArgs &a);
 hipError_t launch_mean(hipStream_t st, const MeanArgs &a);

@@ -8,6 +8,12 @@
 // terms of the D2v and eta equations (:637-661), the implicit operators
 // OS/SQ (:476-477) and the CN-RK3 time scheme (:486), writes the new rhs and
 // replaces the stored explicit term.
+//
+// With a body force (FORCE) the thread also keeps a five-row window of
+// F = F0 + gain * V_src, formed from the velocity rows it already holds, and
+// adds the force terms of buildrhs (dnsdata.f90:643-659) with the wall
+// treatment of :616-628: F is zero on the wall rows and its ghost rows make
+// the compact D4 of the first interior row vanish.
 #include "chan_internal.hpp"
 
 namespace chan {
@@ -15,7 +21,19 @@ namespace chan {
 __device__ __forceinline__ cx<double> imul(double a, cx<double> X) { return cx<double>{-(a * X.y), a * X.x}; }
 __device__ __forceinline__ cx<double> neg(cx<double> X) { return cx<double>{-X.x, -X.y}; }
 
-template <typename R>
+// ghost rows of the body force (dnsdata.f90:617-628): with F(0) = F(-1) = 0 the
+// compact D4 of row 1 is d4(0)F(1) + d4(1)F(2) + d4(2)F(3), and
+// F(-1) = -D4 / d4(-2); at the top F(ny+1) = -D4 / d4(2) of row ny-1.
+__device__ __forceinline__ cx<double> ghost_lo(const double *d4, cx<double> f1, cx<double> f2, cx<double> f3) {
+  const cx<double> s = add(add(scl(d4[2], f1), scl(d4[3], f2)), scl(d4[4], f3));
+  return cx<double>{-s.x / d4[0], -s.y / d4[0]};
+}
+__device__ __forceinline__ cx<double> ghost_hi(const double *d4, cx<double> f3, cx<double> f2, cx<double> f1) {
+  const cx<double> s = add(add(scl(d4[0], f3), scl(d4[1], f2)), scl(d4[2], f1));
+  return cx<double>{-s.x / d4[4], -s.y / d4[4]};
+}
+
+template <typename R, bool FORCE>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restrict__ der,
                                              const DevScalars *__restrict__ sc) {
   // der and sc are separate restrict arguments so that their wave-uniform
@@ -37,7 +55,8 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
   // explicit terms of row iy are loaded one iteration early into nxo,
   // unconditionally (clamped row), so the loads overlap the previous row.
   const size_t rs = (size_t)(g.ny - 1) * ps;  // component stride of old/rhs
-  cx<double> wp[6][5], wv[3][5], nxo[2];
+  cx<double> wp[6][5], wv[3][5], wf[3][5], nxo[2];
+  const int col = mean ? g.Nz : iz + g.nz;  // body-force gain column
   auto load_row = [&](int r, int j) {
     const size_t off = (size_t)(r + 1) * ps + p;
 #pragma unroll
@@ -47,6 +66,20 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) wv[c][j] = a.V[c * pstride + off];
+    if constexpr (FORCE) {
+      const bool inner = r >= 1 && r <= g.ny - 1;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        cx<double> f = {0.0, 0.0};
+        if (a.F0) f = a.F0[c * pstride + off];
+        const int s = a.src[c];
+        if (s >= 0) {
+          const double gn = a.gain[((size_t)c * g.Ny + (r + 1)) * (g.Nz + 1) + col];
+          f = add(f, scl(gn, s == 0 ? wv[0][j] : (s == 1 ? wv[1][j] : wv[2][j])));
+        }
+        wf[c][j] = inner ? f : cx<double>{0.0, 0.0};
+      }
+    }
   };
 #pragma unroll
   for (int j = 0; j < 4; j++) load_row(y0 - 2 + j, j);
@@ -72,15 +105,33 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       for (int j = 1; j < 5; j++) s = add(s, scl(d[f * 5 + j], wp[q][j]));
       return s;
     };
+    // compact stencil f (0: d0, 1: d1) of force component c
+    auto DF = [&](int f, int c) {
+      cx<double> s = scl(d[f * 5 + 0], wf[c][0]);
+#pragma unroll
+      for (int j = 1; j < 5; j++) s = add(s, scl(d[f * 5 + j], wf[c][j]));
+      return s;
+    };
+    if constexpr (FORCE) {
+      if (iy == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) wf[c][0] = ghost_lo(d + 15, wf[c][2], wf[c][3], wf[c][4]);
+      }
+      if (iy == g.ny - 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) wf[c][4] = ghost_hi(d + 15, wf[c][0], wf[c][1], wf[c][2]);
+      }
+    }
     // products: 0 uu, 1 vv, 2 ww, 3 uv, 4 vw, 5 uw
     const cx<double> DD0_6 = DD(0, 5), DD1_6 = DD(1, 5);
     const cx<double> rhsu = sub(sub(neg(imul(al, DD(0, 0))), DD(1, 3)), imul(be, DD0_6));
     const cx<double> rhsv = sub(sub(neg(imul(al, DD(0, 3))), DD(1, 1)), imul(be, DD(0, 4)));
     const cx<double> rhsw = sub(sub(neg(imul(al, DD0_6)), DD(1, 4)), imul(be, DD(0, 2)));
-    const cx<double> expl =
+    cx<double> expl =
         sub(add(imul(al, add(add(imul(al, DD(1, 0)), DD(2, 3)), imul(be, DD1_6))),
                 imul(be, add(add(imul(al, DD1_6), DD(2, 4)), imul(be, DD(1, 2))))),
             scl(k2, rhsv));
+    if constexpr (FORCE) expl = sub(sub(sub(expl, scl(k2, DF(0, 1))), imul(al, DF(1, 0))), imul(be, DF(1, 2)));
     const size_t o = (size_t)(iy - 1) * ps + p;
     // D2v equation: unknown D2(v) - k2 D0(v), implicit OS v
     {
@@ -108,6 +159,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       cx<double> ex, unkn, impl;
       if (mean) {
         ex = cx<double>{rhsu.x + meanpx, rhsw.x + meanpz};
+        if constexpr (FORCE) ex = add(ex, cx<double>{DF(0, 0).x, DF(0, 2).x});
         double a1 = d[0] * wv[0][0].x, a3 = d[0] * wv[2][0].x, b1 = d[10] * wv[0][0].x, b3 = d[10] * wv[2][0].x;
 #pragma unroll
         for (int j = 1; j < 5; j++) {
@@ -120,6 +172,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
         impl = cx<double>{a.ni * b1, a.ni * b3};
       } else {
         ex = sub(imul(be, rhsu), imul(al, rhsw));
+        if constexpr (FORCE) ex = sub(add(ex, imul(be, DF(0, 0))), imul(al, DF(0, 2)));
         cx<double> s1 = scl(d[0], wv[0][0]), s3 = scl(d[0], wv[2][0]);
 #pragma unroll
         for (int j = 1; j < 5; j++) {
@@ -146,14 +199,53 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       for (int q = 0; q < 6; q++) wp[q][j] = wp[q][j + 1];
 #pragma unroll
       for (int c = 0; c < 3; c++) wv[c][j] = wv[c][j + 1];
+      if constexpr (FORCE) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) wf[c][j] = wf[c][j + 1];
+      }
     }
   }
+}
+
+// The body force that buildrhs applies to the resident field, wall rows and
+// ghost rows included, in the device layout (one thread per pair and
+// component).
+__global__ __launch_bounds__(256) void k_force(ForceArgs a, const double *__restrict__ der) {
+  const Geom g = a.g;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npair) return;
+  const int c = blockIdx.y;
+  const int ix = p / g.Nz, iz = p % g.Nz - g.nz;
+  const int col = (ix == 0 && iz == 0) ? g.Nz : iz + g.nz;
+  const size_t ps = g.ps, pstride = (size_t)g.Ny * ps;
+  const int s = a.src[c];
+  auto F = [&](int r) {
+    cx<double> f = {0.0, 0.0};
+    if (r < 1 || r > g.ny - 1) return f;
+    const size_t off = (size_t)(r + 1) * ps + p;
+    if (a.F0) f = a.F0[c * pstride + off];
+    if (s >= 0) f = add(f, scl(a.gain[((size_t)c * g.Ny + (r + 1)) * (g.Nz + 1) + col], a.V[s * pstride + off]));
+    return f;
+  };
+  cx<double> *o = a.out + c * pstride + p;
+  for (int r = 0; r <= g.ny; r++) o[(size_t)(r + 1) * ps] = F(r);
+  o[0] = ghost_lo(der + 15, F(1), F(2), F(3));
+  o[(size_t)(g.ny + 2) * ps] = ghost_hi(der + (size_t)(g.ny - 2) * 20 + 15, F(g.ny - 3), F(g.ny - 2), F(g.ny - 1));
+}
+
+hipError_t launch_force(hipStream_t st, const ForceArgs &a) {
+  dim3 grid((unsigned)((a.g.npair + 255) / 256), 3);
+  hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, a, a.der);
+  return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a) {
   dim3 grid((unsigned)((a.g.npair + 255) / 256), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
-  hipLaunchKernelGGL((k_rhs<R>), grid, dim3(256), 0, st, a, a.der, a.sc);
+  if (a.force)
+    hipLaunchKernelGGL((k_rhs<R, true>), grid, dim3(256), 0, st, a, a.der, a.sc);
+  else
+    hipLaunchKernelGGL((k_rhs<R, false>), grid, dim3(256), 0, st, a, a.der, a.sc);
   return hipGetLastError();
 }
 

@@ -13,6 +13,13 @@ LIBPATH = os.path.join(HERE, "libchannel_hip.so")
 
 CHAN_FP64 = 0
 CHAN_SPLIT = 1
+# status codes below zero (include/channel_hip.h)
+CHAN_E_ARG = -1
+CHAN_E_SIZE = -2
+CHAN_E_NOTFINITE = -3
+CHAN_E_CPITYPE = -4
+CHAN_E_STATE = -5
+CHAN_E_NODEV = -6
 
 _lib = None
 
@@ -66,6 +73,9 @@ SIGNATURES = {
     "chan_linsolve": (ctypes.c_int, [_P, ctypes.c_double]),
     "chan_outstats": (ctypes.c_int, [_P, _D]),
     "chan_stats": (ctypes.c_int, [_P, _D, _D]),
+    "chan_set_body_force": (ctypes.c_int, [_P, _D]),
+    "chan_set_body_force_feedback": (ctypes.c_int, [_P, ctypes.POINTER(_I32), _D]),
+    "chan_get_body_force": (ctypes.c_int, [_P, _D]),
     "chan_step": (ctypes.c_int, [_P, _D]),
     "chan_steps": (ctypes.c_int, [_P, _I32, _D]),
     "chan_timing": (ctypes.c_int, [_P, _I32]),

@@ -115,6 +115,34 @@ class Channel:
         lib.check(self._L.chan_stats(self.h, lib.dptr(a) if a is not None else None, lib.dptr(out)))
         return {"mean": out[0:3], "var": out[3:6], "uv": out[6]}
 
+    # ---------------------------------------------------- body force --
+    def set_body_force(self, F):
+        """Static body force in the layout of V (None removes it)."""
+        if F is None:
+            lib.check(self._L.chan_set_body_force(self.h, None))
+            return
+        F = np.asfortranarray(F, dtype=np.complex128)
+        assert F.shape == self.shape, (F.shape, self.shape)
+        lib.check(self._L.chan_set_body_force(self.h, lib.dptr(F)))
+
+    def set_body_force_feedback(self, src, gain):
+        """F_d += gain[d, iy+1, col] * V[..., src[d]] at every buildrhs; see
+        channel_amd.body_force for the tables of the reference's modules.
+        src=None removes it."""
+        if src is None:
+            lib.check(self._L.chan_set_body_force_feedback(self.h, None, None))
+            return
+        s = (ctypes.c_int32 * 3)(*[int(v) for v in src])
+        g = np.ascontiguousarray(gain, dtype=np.float64)
+        assert g.shape == (3, self.ny + 3, 2 * self.nz + 2), g.shape
+        lib.check(self._L.chan_set_body_force_feedback(self.h, s, lib.dptr(g)))
+
+    def get_body_force(self):
+        """The force the next buildrhs applies (wall and ghost rows included)."""
+        F = np.zeros(self.shape, dtype=np.complex128, order="F")
+        lib.check(self._L.chan_get_body_force(self.h, lib.dptr(F)))
+        return F
+
     def steps(self, n):
         r = np.zeros((n, 11))
         if n:

@@ -148,6 +148,36 @@ int chan_steps(chan_ctx *c, int32_t n, double *rows);
  * is NULL, the field's own plane mean (instantaneous fluctuations). */
 int chan_stats(chan_ctx *c, const double *avg, double *out);
 
+/* Body force (the reference's compile-time `bodyforce` option,
+ * dnsdata.f90:51-54, 133, 616-628, 643-659).  buildrhs adds the force
+ * F(-1:ny+1,-nz:nz,0:nx,1:3) to the explicit terms: -k2 D0(F_v) - i kx D1(F_u)
+ * - i kz D1(F_w) in the D2v equation, i kz D0(F_u) - i kx D0(F_w) in the eta
+ * equation and (D0 Re F_u, D0 Re F_w) in the mean-flow mode.  As in the
+ * reference, F is zero on the wall rows 0 and ny and its ghost rows -1, ny+1
+ * make the compact D4 of rows 1 and ny-1 vanish, whatever the caller supplies
+ * there.  The force is the sum of two optional parts:
+ *
+ * chan_set_body_force: a static field F0 in the layout of V (NULL removes it);
+ * the role of a config_body_force that fills F once.
+ *
+ * chan_set_body_force_feedback: F_d += gain[d][iy+1][col] * V_src[d], formed
+ * from the field at the start of every buildrhs -- the reference's
+ * set_body_force called before each substep (channel.f90:129-157).  src[d] in
+ * {-1 (none), 0 (u), 1 (v), 2 (w)}; gain is [3][ny+3][2nz+2] doubles with
+ * col = iz+nz for every mode except (ix,iz) = (0,0), which reads col = 2nz+1.
+ * This covers body_forces/am_f1 and am_butterfly (src = {0,1,2}, gain -amp on
+ * their masks) and body_forces/coriolis (src = {1,0,-1}, gains -omega2 and
+ * +omega2); channel_amd.body_force builds those tables.  src or gain NULL, or
+ * every src -1, removes it.
+ *
+ * chan_get_body_force: the force that the next buildrhs applies to the
+ * resident field, wall and ghost rows included, in the layout of V (zero
+ * when no force is set); the payload of the reference's Force.cart.N.out
+ * (dnsdata.f90:904-906) for a static force. */
+int chan_set_body_force(chan_ctx *c, const double *F);
+int chan_set_body_force_feedback(chan_ctx *c, const int32_t src[3], const double *gain);
+int chan_get_body_force(chan_ctx *c, double *F);
+
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch (off by default).  chan_kernel_times fills up to cap entries of
  * names (NUL-terminated, 32 chars each), launch counts and total ms, and

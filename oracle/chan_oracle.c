@@ -21,6 +21,12 @@
  *                         growth rate of the leading mode (independent
  *                         Chebyshev solver in the tests)
  *   time loop/outstats -- Runtimedata columns of a laminar run (closed form)
+ *   body force         -- laminar Poiseuille kept by a uniform force instead
+ *                         of dp/dx (u and w); steady forced Stokes flow of
+ *                         oblique and 2-D modes against a Chebyshev solution
+ *                         of the forced v-eta equations; damping of an
+ *                         Orr-Sommerfeld wave by exactly amp under am_f1;
+ *                         module masks against an independent table builder
  */
 #include "chan_oracle.h"
 
@@ -263,6 +269,10 @@ struct orc {
   BCOND *bc0, *bcn;   /* [ix][iz] */
   double time, deltat, meanpx, meanpz, corrpx, corrpz, fr[3], cfl;
   fftplan pz, px;
+  cplx *F;          /* body force, layout of V; NULL when off */
+  int bf_kind;      /* 0 none, 1 am_f1, 2 am_butterfly, 3 coriolis */
+  double bf_par[3];
+  int bf_iz;        /* iz_f (am_*) or iz_thr (coriolis) */
 };
 
 static int g_threads = 1;
@@ -444,6 +454,7 @@ void orc_destroy(orc_t *o) {
   free(o->P);
   free(o->bc0);
   free(o->bcn);
+  free(o->F);
   plan_free(&o->pz);
   plan_free(&o->px);
   free(o);
@@ -696,6 +707,24 @@ static void rhs_rows(orc_t *o, const double ode[3], int lo, int hi, cplx *newrhs
         cplx rhsw = -imul(al, DD0_6) - DD(d1, 5) - imul(be, DD(d0, 3));
         cplx expl = imul(al, imul(al, DD(d1, 1)) + DD(d2, 4) + imul(be, DD1_6)) +
                     imul(be, imul(al, DD1_6) + DD(d2, 5) + imul(be, DD(d1, 3))) - k2 * rhsv;
+        /* body-force terms (dnsdata.f90:643-645, 650-652, 657-659): D0 and
+         * D1 of F(iy-2:iy+2) by the D0/D1 macros (:330-331) */
+        cplx D0F[3] = {0, 0, 0}, D1F[3] = {0, 0, 0};
+        if (o->F) {
+          for (int c = 0; c < 3; c++) {
+            double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+            for (int j = 0; j < 5; j++) {
+              cplx f = o->F[VIX(o, iy - 2 + j, iz, ix, c)];
+              r0 += d->d0[j] * creal(f);
+              i0 += d->d0[j] * cimag(f);
+              r1 += d->d1[j] * creal(f);
+              i1 += d->d1[j] * cimag(f);
+            }
+            D0F[c] = CMPLX(r0, i0);
+            D1F[c] = CMPLX(r1, i1);
+          }
+          expl = expl - k2 * D0F[1] - imul(al, D1F[0]) - imul(be, D1F[2]);
+        }
         size_t oi = OIX(o, iy, ix, iz);
         size_t ni_ = (((size_t)(iy - 1) * Nx + ix) * Nz + (iz + nz)) * 2;
         /* D2v equation */
@@ -721,6 +750,7 @@ static void rhs_rows(orc_t *o, const double ode[3], int lo, int hi, cplx *newrhs
         }
         if (ix == 0 && iz == 0) {
           cplx ex = CMPLX(creal(rhsu) + o->meanpx, creal(rhsw) + o->meanpz);
+          if (o->F) ex = ex + CMPLX(creal(D0F[0]), creal(D0F[2])); /* rD0(F,1,3) */
           double a1 = 0, a3 = 0, b1 = 0, b3 = 0;
           for (int j = 0; j < 5; j++) {
             a1 += d->d0[j] * creal(VV(j, 1));
@@ -737,6 +767,7 @@ static void rhs_rows(orc_t *o, const double ode[3], int lo, int hi, cplx *newrhs
           o->oldrhs[oi + 0] = ex;
         } else {
           cplx ex = imul(be, rhsu) - imul(al, rhsw);
+          if (o->F) ex = ex + imul(be, D0F[0]) - imul(al, D0F[2]);
           double r1 = 0, i1 = 0, r3 = 0, i3 = 0;
           for (int j = 0; j < 5; j++) {
             r1 += d->d0[j] * creal(VV(j, 1));
@@ -763,8 +794,122 @@ static void rhs_rows(orc_t *o, const double ode[3], int lo, int hi, cplx *newrhs
   }
 }
 
+/* ---------------------------------------------------------- body force -- */
+/* The three modules of body_forces/, each a set_body_force that writes
+ * F(iy,iz,ix,pc) = g V(iy,iz,ix,c) where its mask holds and leaves F alone
+ * elsewhere. */
+static void bf_am(orc_t *o, int butterfly) {
+  /* am_f1.inc:11-27 and am_butterfly.inc:11-27; amp = bf_par[0].  am_f1
+   * loops over iz = -iz_f..iz_f, capped here at nz (the last mode). */
+  const double amp = o->bf_par[0];
+  const int nz = o->nz, izf = o->bf_iz;
+  const int lo = butterfly ? -nz : -(izf < nz ? izf : nz);
+  for (int iV = 0; iV < 3; iV++)
+    for (int ix = 0; ix <= o->nx; ix++)
+      for (int iz = lo; iz <= -lo; iz++) {
+        double lzp = iz == 0 ? (double)1E10f : 2 * o->PI / (o->p.beta0 * abs(iz)) * 1000;
+        for (int iy = -1; iy <= o->ny + 1; iy++) {
+          double yp = (Y(o, iy) > 1 ? o->p.ymax - Y(o, iy) : Y(o, iy)) * 1000;
+          int on;
+          if (!butterfly)
+            on = lzp > (double)2.3f * (yp * yp) && !(ix == 0 && iz == 0);
+          else
+            on = (abs(iz) <= izf && yp <= 60 && !(ix == 0 && iz == 0)) || (abs(iz) > izf && yp > 60);
+          if (on) o->F[VIX(o, iy, iz, ix, iV)] = -(amp * o->V[VIX(o, iy, iz, ix, iV)]);
+        }
+      }
+}
+
+static void bf_coriolis(orc_t *o) {
+  /* coriolis.inc:26-40: omega2, y_threshold_bot = bf_par[0], bf_par[2] */
+  const double omega2 = o->bf_par[0], bot = o->bf_par[2], top = o->p.ymax - bot;
+  for (int c = 1; c <= 2; c++) {
+    int pc = c % 2 + 1, zeichen = 2 * pc - 3;
+    for (int ix = 0; ix <= o->nx; ix++)
+      for (int iz = -o->bf_iz; iz <= o->bf_iz; iz++)
+        for (int iy = -1; iy <= o->ny + 1; iy++)
+          if (Y(o, iy) <= bot || Y(o, iy) >= top)
+            o->F[VIX(o, iy, iz, ix, pc - 1)] = (zeichen * omega2) * o->V[VIX(o, iy, iz, ix, c - 1)];
+  }
+}
+
+static void bf_alloc(orc_t *o) {
+  if (!o->F) o->F = (cplx *)calloc((size_t)o->Ny * o->Nz * o->Nx * 3, sizeof(cplx));
+}
+
+void orc_set_body_force(orc_t *o, const double *F) {
+  if (!F) {
+    if (!o->bf_kind) {
+      free(o->F);
+      o->F = NULL;
+    }
+    return;
+  }
+  bf_alloc(o);
+  memcpy(o->F, F, sizeof(cplx) * (size_t)o->Ny * o->Nz * o->Nx * 3);
+}
+
+void orc_set_body_force_module(orc_t *o, int kind, const double *par) {
+  o->bf_kind = kind;
+  if (!kind) return;
+  bf_alloc(o);
+  memcpy(o->bf_par, par, sizeof o->bf_par);
+  if (kind == 1 || kind == 2) /* iz_f = NINT((2 PI / lambdaz_f) / (beta0/1000)), am_f1.inc:7 */
+    o->bf_iz = (int)lround((2.0 * o->PI / par[1]) / (o->p.beta0 / 1000.0));
+  else { /* iz_thr = min(nz, floor(kz_cutoff / beta0)), coriolis.inc:15 */
+    double t = floor(par[1] / o->p.beta0);
+    o->bf_iz = t < o->nz ? (int)t : o->nz;
+  }
+}
+
+void orc_apply_body_force(orc_t *o) {
+  if (o->bf_kind == 1 || o->bf_kind == 2) bf_am(o, o->bf_kind == 2);
+  if (o->bf_kind == 3) bf_coriolis(o);
+}
+
+void orc_get_body_force(const orc_t *o, double *F) {
+  size_t n = (size_t)o->Ny * o->Nz * o->Nx * 3;
+  if (o->F)
+    memcpy(F, o->F, sizeof(cplx) * n);
+  else
+    memset(F, 0, sizeof(cplx) * n);
+}
+
+/* the wall rows of F at the start of buildrhs (dnsdata.f90:616-628): zero on
+ * the walls, ghost rows from D4(F) = 0 on rows 1 and ny-1 (D4 macro :333) */
+void orc_body_force_walls(orc_t *o) {
+  if (!o->F) return;
+  const int ny = o->ny;
+  for (int c = 0; c < 3; c++)
+    for (int ix = 0; ix <= o->nx; ix++)
+      for (int iz = -o->nz; iz <= o->nz; iz++) {
+        o->F[VIX(o, -1, iz, ix, c)] = 0;
+        o->F[VIX(o, 0, iz, ix, c)] = 0;
+        o->F[VIX(o, ny, iz, ix, c)] = 0;
+        o->F[VIX(o, ny + 1, iz, ix, c)] = 0;
+      }
+  for (int side = 0; side < 2; side++) {
+    const int iy = side ? ny - 1 : 1, g = side ? ny + 1 : -1;
+    const double *d4 = o->der[iy].d4, piv = side ? d4[4] : d4[0];
+    for (int c = 0; c < 3; c++)
+      for (int ix = 0; ix <= o->nx; ix++)
+        for (int iz = -o->nz; iz <= o->nz; iz++) {
+          double re = 0, im = 0;
+          for (int j = 0; j < 5; j++) {
+            cplx f = o->F[VIX(o, iy - 2 + j, iz, ix, c)];
+            re += d4[j] * creal(f);
+            im += d4[j] * cimag(f);
+          }
+          o->F[VIX(o, g, iz, ix, c)] = CMPLX(-re / piv, -im / piv);
+        }
+  }
+}
+
 void orc_buildrhs(orc_t *o, const double ode[3], int compute_cfl) {
   int ny = o->ny, nx = o->nx, nz = o->nz, Nz = o->Nz, Nx = o->Nx;
+  /* set_body_force runs right before every buildrhs (channel.f90:129-157) */
+  orc_apply_body_force(o);
+  orc_body_force_walls(o);
   all_products(o, compute_cfl, -1, ny + 1);
   cplx *newrhs = (cplx *)malloc(sizeof(cplx) * (size_t)(ny - 1) * Nx * Nz * 2);
   rhs_rows(o, ode, 1, ny - 1, newrhs);

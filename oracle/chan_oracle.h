@@ -64,6 +64,21 @@ void orc_step(orc_t *o, double row[11]);
 void orc_steps(orc_t *o, int n, double *rows);
 void orc_time_sample(orc_t *o, int nplanes, int nix, double out[3]);
 
+/* body force (the bodyforce option, dnsdata.f90:51-54, 616-628, 643-659).
+ * orc_set_body_force copies a static F in the layout of V (NULL removes it
+ * unless a module is set).  orc_set_body_force_module selects a module of
+ * body_forces/ that runs at the start of every orc_buildrhs, as the
+ * reference calls set_body_force before every buildrhs: kind 0 none,
+ * 1 am_f1 and 2 am_butterfly with par = {amp, lambdaz_f, -}, 3 coriolis with
+ * par = {omega2, kz_cutoff, y_threshold_bot}.  A module writes over F where
+ * its mask holds.  orc_apply_body_force runs the module once;
+ * orc_body_force_walls applies buildrhs's wall rows. */
+void orc_set_body_force(orc_t *o, const double *F);
+void orc_set_body_force_module(orc_t *o, int kind, const double *par);
+void orc_apply_body_force(orc_t *o);
+void orc_body_force_walls(orc_t *o);
+void orc_get_body_force(const orc_t *o, double *F);
+
 /* primitives exposed for pinning tests */
 void orc_fft(int n, int sign, double *inout);
 void orc_c2r(int m, const double *X, double *x);

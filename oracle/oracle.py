@@ -76,6 +76,11 @@ def lib():
         L.orc_yintegr.restype = ctypes.c_double
         L.orc_yintegr.argtypes = [P, D]
         L.orc_complexderiv.argtypes = [P, D, D]
+        L.orc_set_body_force.argtypes = [P, D]
+        L.orc_set_body_force_module.argtypes = [P, I, D]
+        L.orc_apply_body_force.argtypes = [P]
+        L.orc_body_force_walls.argtypes = [P]
+        L.orc_get_body_force.argtypes = [P, D]
         _LIB = L
     return _LIB
 
@@ -193,6 +198,35 @@ class Oracle:
         r = np.zeros((n, 11))
         lib().orc_steps(self.h, int(n), _dp(r))
         return r
+
+    # body force
+    MODULES = {"am_f1": 1, "am_butterfly": 2, "coriolis": 3}
+
+    def set_body_force(self, F):
+        if F is None:
+            lib().orc_set_body_force(self.h, None)
+            return
+        F = np.asfortranarray(F, dtype=np.complex128)
+        assert F.shape == self.shape
+        lib().orc_set_body_force(self.h, _dp(F))
+
+    def set_body_force_module(self, name, *par):
+        """am_f1/am_butterfly(amp, lambdaz_f) or coriolis(omega2, kz_cutoff,
+        y_threshold_bot); name None switches the module off."""
+        p = np.zeros(3)
+        p[:len(par)] = par
+        lib().orc_set_body_force_module(self.h, self.MODULES[name] if name else 0, _dp(p))
+
+    def apply_body_force(self):
+        lib().orc_apply_body_force(self.h)
+
+    def body_force_walls(self):
+        lib().orc_body_force_walls(self.h)
+
+    def get_body_force(self):
+        F = np.zeros(self.shape, dtype=np.complex128, order="F")
+        lib().orc_get_body_force(self.h, _dp(F))
+        return F
 
     def time_sample(self, nplanes, nix):
         """Seconds per plane of products, per row of rhs, per x-mode column
