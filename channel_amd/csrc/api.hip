@@ -237,6 +237,19 @@ int dalloc(T **p, size_t n) {
   return 0;
 }
 
+ForceArgs force_args(const chan_ctx *c) {
+  ForceArgs a;
+  a.V = c->V;
+  a.F0 = c->F0;
+  a.gain = c->gain;
+  a.der = c->der;
+  for (int k = 0; k < 3; k++) a.src[k] = c->gain ? c->src[k] : -1;
+  a.out = nullptr;
+  a.g = c->g;
+  a.chunk = 48;
+  return a;
+}
+
 double cfl_value(const DevScalars &s) {
   double v;
   std::memcpy(&v, &s.cfl_bits, sizeof v);
@@ -483,14 +496,8 @@ int chan_get_body_force(chan_ctx *c, double *F) {
   // the products buffer (dead between calls; 6*Ny*ps elements of at least
   // 8 bytes) holds the force in the device layout, 3*Ny*ps complex FP64
   cx<double> *dev = static_cast<cx<double> *>(c->P);
-  ForceArgs a;
-  a.V = c->V;
-  a.F0 = c->F0;
-  a.gain = c->gain;
-  a.der = c->der;
-  for (int k = 0; k < 3; k++) a.src[k] = c->gain ? c->src[k] : -1;
+  ForceArgs a = force_args(c);
   a.out = dev;
-  a.g = c->g;
   HIPCHK(launch_force(c->st, a));
   HIPCHK(launch_to_host_layout(c->st, dev, stage, c->g));
   HIPCHK(hipMemcpyAsync(F, stage, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
@@ -602,10 +609,13 @@ static int rhs_t(chan_ctx *c, const double ode[3]) {
   a.dt = c->deltat;
   a.ni = c->prm.ni;
   a.chunk = 48;
-  a.F0 = c->F0;
-  a.gain = c->gain;
-  for (int k = 0; k < 3; k++) a.src[k] = c->gain ? c->src[k] : -1;
   a.force = c->F0 || c->gain;
+  if (a.force) {
+    ForceArgs f = force_args(c);
+    f.out = c->rhs;
+    int rc = timed(c, "force", [&] { return launch_force_terms(c->st, f); });
+    if (rc) return rc;
+  }
   return timed(c, "rhs", [&] { return launch_rhs<R>(c->st, a); });
 }
 }  // extern "C++"

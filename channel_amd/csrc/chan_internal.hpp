@@ -73,23 +73,21 @@ struct RhsArgs {
   cx<double> *rhs;       // [2][ny-1][ps]
   const double *der;     // [(ny-1)][4][5]
   const DevScalars *sc;  // meanpx, meanpz
-  // body force F_d = F0_d + gain[d][plane][col] * V_src[d] (either part may be
-  // absent: F0 null, src[d] < 0); col = iz+nz, or Nz for the mode (0,0)
-  const cx<double> *F0;  // [3][Ny][ps] or null
-  const double *gain;    // [3][Ny][Nz+1] or null
-  int src[3];
-  bool force;
+  bool force;            // rhs holds the body-force terms (k_force_terms) on entry
   Geom g;
   double ode0, ode1, ode2, dt, ni;
   int chunk;
 };
 
+// Body force F_d = F0_d + gain[d][plane][col] * V_src[d]; either part may be
+// absent (F0 null, src[d] < 0).  col = iz+nz, or Nz for the mode (0,0).
 struct ForceArgs {
-  const cx<double> *V, *F0;
-  const double *gain, *der;
+  const cx<double> *V, *F0;  // [3][Ny][ps]; F0 may be null
+  const double *gain, *der;  // gain [3][Ny][Nz+1]
   int src[3];
-  cx<double> *out;  // [3][Ny][ps]
+  cx<double> *out;  // k_force: [3][Ny][ps]; k_force_terms: [2][ny-1][ps]
   Geom g;
+  int chunk;
 };
 
 struct SolveArgs {
@@ -143,6 +141,7 @@ hipError_t launch_fft_bench(int N, int sign, int variant, const cx<double> *in, 
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a);
 hipError_t launch_force(hipStream_t st, const ForceArgs &a);
+hipError_t launch_force_terms(hipStream_t st, const ForceArgs &a);
 hipError_t launch_stats(hipStream_t st, const cx<double> *V, const double *avg, double *out, const Geom &g);
 hipError_t launch_solve(hipStream_t st, const SolveArgs &a);
 hipError_t launch_mean(hipStream_t st, const MeanArgs &a);

@@ -9,11 +9,13 @@
 // OS/SQ (:476-477) and the CN-RK3 time scheme (:486), writes the new rhs and
 // replaces the stored explicit term.
 //
-// With a body force (FORCE) the thread also keeps a five-row window of
-// F = F0 + gain * V_src, formed from the velocity rows it already holds, and
-// adds the force terms of buildrhs (dnsdata.f90:643-659) with the wall
-// treatment of :616-628: F is zero on the wall rows and its ghost rows make
-// the compact D4 of the first interior row vanish.
+// With a body force, k_force_terms first forms the force terms of buildrhs
+// (dnsdata.f90:643-659) from F = F0 + gain * V_src with the wall treatment
+// of :616-628 (F zero on the wall rows, ghost rows that make the compact D4
+// of the first and last interior rows vanish) and leaves them in the rhs
+// buffer; the FORCE variant of k_rhs adds them to the explicit terms as it
+// overwrites that buffer.  A five-row force window inside k_rhs itself would
+// push it past the register file.
 #include "chan_internal.hpp"
 
 namespace chan {
@@ -55,8 +57,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
   // explicit terms of row iy are loaded one iteration early into nxo,
   // unconditionally (clamped row), so the loads overlap the previous row.
   const size_t rs = (size_t)(g.ny - 1) * ps;  // component stride of old/rhs
-  cx<double> wp[6][5], wv[3][5], wf[3][5], nxo[2];
-  const int col = mean ? g.Nz : iz + g.nz;  // body-force gain column
+  cx<double> wp[6][5], wv[3][5], nxo[2], nxf[2];
   auto load_row = [&](int r, int j) {
     const size_t off = (size_t)(r + 1) * ps + p;
 #pragma unroll
@@ -66,20 +67,6 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) wv[c][j] = a.V[c * pstride + off];
-    if constexpr (FORCE) {
-      const bool inner = r >= 1 && r <= g.ny - 1;
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        cx<double> f = {0.0, 0.0};
-        if (a.F0) f = a.F0[c * pstride + off];
-        const int s = a.src[c];
-        if (s >= 0) {
-          const double gn = a.gain[((size_t)c * g.Ny + (r + 1)) * (g.Nz + 1) + col];
-          f = add(f, scl(gn, s == 0 ? wv[0][j] : (s == 1 ? wv[1][j] : wv[2][j])));
-        }
-        wf[c][j] = inner ? f : cx<double>{0.0, 0.0};
-      }
-    }
   };
 #pragma unroll
   for (int j = 0; j < 4; j++) load_row(y0 - 2 + j, j);
@@ -89,14 +76,28 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
   };
   nxo[0] = ldold((size_t)(y0 - 1) * ps + p);
   nxo[1] = ldold(rs + (size_t)(y0 - 1) * ps + p);
+  // force terms of the eta and D2v equations, left in rhs by k_force_terms
+  if constexpr (FORCE) {
+    nxf[0] = a.rhs[(size_t)(y0 - 1) * ps + p];
+    nxf[1] = a.rhs[rs + (size_t)(y0 - 1) * ps + p];
+  }
 
   for (int iy = y0; iy <= y1; iy++) {
     load_row(iy + 2, 4);
     const cx<double> old_eta = nxo[0], old_v = nxo[1];
+    cx<double> f_eta = {0.0, 0.0}, f_v = {0.0, 0.0};
+    if constexpr (FORCE) {
+      f_eta = nxf[0];
+      f_v = nxf[1];
+    }
     {
       const int on = min(iy, g.ny - 2);
       nxo[0] = ldold((size_t)on * ps + p);
       nxo[1] = ldold(rs + (size_t)on * ps + p);
+      if constexpr (FORCE) {
+        nxf[0] = a.rhs[(size_t)on * ps + p];
+        nxf[1] = a.rhs[rs + (size_t)on * ps + p];
+      }
     }
     const double *d = der + (size_t)(iy - 1) * 20;  // d0[5] d1[5] d2[5] d4[5]
     auto DD = [&](int f, int q) {
@@ -105,23 +106,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       for (int j = 1; j < 5; j++) s = add(s, scl(d[f * 5 + j], wp[q][j]));
       return s;
     };
-    // compact stencil f (0: d0, 1: d1) of force component c
-    auto DF = [&](int f, int c) {
-      cx<double> s = scl(d[f * 5 + 0], wf[c][0]);
-#pragma unroll
-      for (int j = 1; j < 5; j++) s = add(s, scl(d[f * 5 + j], wf[c][j]));
-      return s;
-    };
-    if constexpr (FORCE) {
-      if (iy == 1) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) wf[c][0] = ghost_lo(d + 15, wf[c][2], wf[c][3], wf[c][4]);
-      }
-      if (iy == g.ny - 1) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) wf[c][4] = ghost_hi(d + 15, wf[c][0], wf[c][1], wf[c][2]);
-      }
-    }
+
     // products: 0 uu, 1 vv, 2 ww, 3 uv, 4 vw, 5 uw
     const cx<double> DD0_6 = DD(0, 5), DD1_6 = DD(1, 5);
     const cx<double> rhsu = sub(sub(neg(imul(al, DD(0, 0))), DD(1, 3)), imul(be, DD0_6));
@@ -131,7 +116,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
         sub(add(imul(al, add(add(imul(al, DD(1, 0)), DD(2, 3)), imul(be, DD1_6))),
                 imul(be, add(add(imul(al, DD1_6), DD(2, 4)), imul(be, DD(1, 2))))),
             scl(k2, rhsv));
-    if constexpr (FORCE) expl = sub(sub(sub(expl, scl(k2, DF(0, 1))), imul(al, DF(1, 0))), imul(be, DF(1, 2)));
+    if constexpr (FORCE) expl = add(expl, f_v);
     const size_t o = (size_t)(iy - 1) * ps + p;
     // D2v equation: unknown D2(v) - k2 D0(v), implicit OS v
     {
@@ -159,7 +144,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       cx<double> ex, unkn, impl;
       if (mean) {
         ex = cx<double>{rhsu.x + meanpx, rhsw.x + meanpz};
-        if constexpr (FORCE) ex = add(ex, cx<double>{DF(0, 0).x, DF(0, 2).x});
+        if constexpr (FORCE) ex = add(ex, f_eta);
         double a1 = d[0] * wv[0][0].x, a3 = d[0] * wv[2][0].x, b1 = d[10] * wv[0][0].x, b3 = d[10] * wv[2][0].x;
 #pragma unroll
         for (int j = 1; j < 5; j++) {
@@ -172,7 +157,7 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
         impl = cx<double>{a.ni * b1, a.ni * b3};
       } else {
         ex = sub(imul(be, rhsu), imul(al, rhsw));
-        if constexpr (FORCE) ex = sub(add(ex, imul(be, DF(0, 0))), imul(al, DF(0, 2)));
+        if constexpr (FORCE) ex = add(ex, f_eta);
         cx<double> s1 = scl(d[0], wv[0][0]), s3 = scl(d[0], wv[2][0]);
 #pragma unroll
         for (int j = 1; j < 5; j++) {
@@ -199,11 +184,76 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs<R> a, const double *__restr
       for (int q = 0; q < 6; q++) wp[q][j] = wp[q][j + 1];
 #pragma unroll
       for (int c = 0; c < 3; c++) wv[c][j] = wv[c][j + 1];
-      if constexpr (FORCE) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) wf[c][j] = wf[c][j + 1];
-      }
     }
+  }
+}
+
+// F_c at interior row r (1 <= r <= ny-1) of pair p: the static part plus the
+// feedback gain times the source velocity component.
+__device__ __forceinline__ cx<double> force_at(const ForceArgs &a, int c, int s, int r, int p, int col) {
+  const Geom &g = a.g;
+  const size_t pstride = (size_t)g.Ny * g.ps, off = (size_t)(r + 1) * g.ps + p;
+  cx<double> f = {0.0, 0.0};
+  if (a.F0) f = a.F0[c * pstride + off];
+  if (s >= 0) f = add(f, scl(a.gain[((size_t)c * g.Ny + (r + 1)) * (g.Nz + 1) + col], a.V[s * pstride + off]));
+  return f;
+}
+
+// F of row r with the wall rows of buildrhs: zero on rows 0 and ny, ghost
+// rows -1 and ny+1 from the compact D4 of rows 1 and ny-1.
+__device__ __forceinline__ cx<double> force_row(const ForceArgs &a, const double *der, int c, int s, int r, int p,
+                                                int col) {
+  const int ny = a.g.ny;
+  if (r >= 1 && r <= ny - 1) return force_at(a, c, s, r, p, col);
+  if (r == -1) return ghost_lo(der + 15, force_at(a, c, s, 1, p, col), force_at(a, c, s, 2, p, col), force_at(a, c, s, 3, p, col));
+  if (r == ny + 1)
+    return ghost_hi(der + (size_t)(ny - 2) * 20 + 15, force_at(a, c, s, ny - 3, p, col), force_at(a, c, s, ny - 2, p, col),
+                    force_at(a, c, s, ny - 1, p, col));
+  return cx<double>{0.0, 0.0};
+}
+
+// The force terms of buildrhs for rows y0..y1 of pair p (one thread per pair
+// and chunk of rows, five-row window of F):
+//   out[0] (eta)  i kz D0(F_u) - i kx D0(F_w), or (D0 Re F_u, D0 Re F_w) in
+//                 the mode (0,0) (rD0, dnsdata.f90:650-652, 657-659);
+//   out[1] (D2v)  -k2 D0(F_v) - i kx D1(F_u) - i kz D1(F_w) (:643-645).
+__global__ __launch_bounds__(256) void k_force_terms(ForceArgs a, const double *__restrict__ der) {
+  const Geom g = a.g;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= g.npair) return;
+  const int ix = p / g.Nz, iz = p % g.Nz - g.nz;
+  const bool mean = (ix == 0 && iz == 0);
+  const int col = mean ? g.Nz : iz + g.nz;
+  const double al = g.alfa0 * ix, be = g.beta0 * iz;
+  const double k2 = (g.alfa0 * ix) * (g.alfa0 * ix) + (g.beta0 * iz) * (g.beta0 * iz);
+  const int y0 = 1 + (int)blockIdx.y * a.chunk;
+  if (y0 > g.ny - 1) return;
+  const int y1 = min(g.ny - 1, y0 + a.chunk - 1);
+  const size_t ps = g.ps, rs = (size_t)(g.ny - 1) * ps;
+  cx<double> w[3][5];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) w[c][j] = force_row(a, der, c, a.src[c], y0 - 2 + j, p, col);
+  for (int iy = y0; iy <= y1; iy++) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) w[c][4] = force_row(a, der, c, a.src[c], iy + 2, p, col);
+    const double *d = der + (size_t)(iy - 1) * 20;
+    auto S = [&](int f, int c) {
+      cx<double> s = scl(d[f * 5 + 0], w[c][0]);
+#pragma unroll
+      for (int j = 1; j < 5; j++) s = add(s, scl(d[f * 5 + j], w[c][j]));
+      return s;
+    };
+    const cx<double> d0u = S(0, 0), d0w = S(0, 2);
+    const cx<double> te = mean ? cx<double>{d0u.x, d0w.x} : sub(imul(be, d0u), imul(al, d0w));
+    const cx<double> tv = sub(sub(neg(scl(k2, S(0, 1))), imul(al, S(1, 0))), imul(be, S(1, 2)));
+    a.out[(size_t)(iy - 1) * ps + p] = te;
+    a.out[rs + (size_t)(iy - 1) * ps + p] = tv;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) w[c][j] = w[c][j + 1];
   }
 }
 
@@ -217,25 +267,19 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const double *__rest
   const int c = blockIdx.y;
   const int ix = p / g.Nz, iz = p % g.Nz - g.nz;
   const int col = (ix == 0 && iz == 0) ? g.Nz : iz + g.nz;
-  const size_t ps = g.ps, pstride = (size_t)g.Ny * ps;
-  const int s = a.src[c];
-  auto F = [&](int r) {
-    cx<double> f = {0.0, 0.0};
-    if (r < 1 || r > g.ny - 1) return f;
-    const size_t off = (size_t)(r + 1) * ps + p;
-    if (a.F0) f = a.F0[c * pstride + off];
-    if (s >= 0) f = add(f, scl(a.gain[((size_t)c * g.Ny + (r + 1)) * (g.Nz + 1) + col], a.V[s * pstride + off]));
-    return f;
-  };
-  cx<double> *o = a.out + c * pstride + p;
-  for (int r = 0; r <= g.ny; r++) o[(size_t)(r + 1) * ps] = F(r);
-  o[0] = ghost_lo(der + 15, F(1), F(2), F(3));
-  o[(size_t)(g.ny + 2) * ps] = ghost_hi(der + (size_t)(g.ny - 2) * 20 + 15, F(g.ny - 3), F(g.ny - 2), F(g.ny - 1));
+  cx<double> *o = a.out + c * (size_t)g.Ny * g.ps + p;
+  for (int r = -1; r <= g.ny + 1; r++) o[(size_t)(r + 1) * g.ps] = force_row(a, der, c, a.src[c], r, p, col);
 }
 
 hipError_t launch_force(hipStream_t st, const ForceArgs &a) {
   dim3 grid((unsigned)((a.g.npair + 255) / 256), 3);
   hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, a, a.der);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_terms(hipStream_t st, const ForceArgs &a) {
+  dim3 grid((unsigned)((a.g.npair + 255) / 256), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
+  hipLaunchKernelGGL(k_force_terms, grid, dim3(256), 0, st, a, a.der);
   return hipGetLastError();
 }
 
