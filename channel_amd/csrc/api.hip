@@ -831,7 +831,7 @@ int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout) {
 }
 
 int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int32_t reps, double *inout, double *ms) {
-  const int inner = 1 + (variant >> 3);  // transforms per load/store pair
+  const int inner = 1 + (variant >> 4);  // transforms per load/store pair
   if (!inout || !ms || batch < 1 || reps < 1 || variant < 0 || inner > 64 || (sign != 1 && sign != -1))
     return fail(CHAN_E_ARG, "bad argument");
   if (n != 384 && n != 768 && n != 1536) return fail(CHAN_E_SIZE, "size %d not benchmarked", n);
@@ -847,9 +847,9 @@ int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int3
   if (e == hipSuccess) e = hipMemcpy(t, tw.data(), sizeof(cx<double>) * tw.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  if (e == hipSuccess) e = launch_fft_bench(n, sign, variant & 7, a, b, t, batch, 1);
+  if (e == hipSuccess) e = launch_fft_bench(n, sign, variant & 15, a, b, t, batch, 1);
   if (e == hipSuccess) e = hipEventRecord(e0, 0);
-  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_fft_bench(n, sign, variant & 7, a, b, t, batch, inner);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_fft_bench(n, sign, variant & 15, a, b, t, batch, inner);
   if (e == hipSuccess) e = hipEventRecord(e1, 0);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   float ft = 0;

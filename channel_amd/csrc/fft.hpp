@@ -230,17 +230,29 @@ __device__ __forceinline__ void dft(cx<R> *a) {
 // reals per team), otherwise real and imaginary parts travel in two rounds
 // (N reals per team).  XCH_PAD inserts one element after every 32, which
 // breaks the power-of-two strides of the Stockham scatter across LDS banks.
-constexpr int XCH_CPLX = 1, XCH_PAD = 2;
+// The other layouts (padding after every 16, 8 or 4 elements, or an XOR of
+// the low three index bits with the next three) exist for timing.
+constexpr int XCH_CPLX = 1, XCH_PAD = 2, XCH_PAD16 = 8, XCH_PAD8 = 16, XCH_PAD4 = 32, XCH_XOR = 64;
 
 template <int XCH>
 __device__ __forceinline__ int xidx(int i) {
-  return (XCH & XCH_PAD) ? i + (i >> 5) : i;
+  if constexpr (XCH & XCH_PAD) return i + (i >> 5);
+  if constexpr (XCH & XCH_PAD16) return i + (i >> 4);
+  if constexpr (XCH & XCH_PAD8) return i + (i >> 3);
+  if constexpr (XCH & XCH_PAD4) return i + (i >> 2);
+  if constexpr (XCH & XCH_XOR) return i ^ ((i >> 3) & 7);
+  return i;
+}
+
+template <int N, int XCH>
+constexpr int xch_len() {
+  return (XCH & XCH_PAD) ? N + N / 32 : (XCH & XCH_PAD16) ? N + N / 16 : (XCH & XCH_PAD8) ? N + N / 8 : (XCH & XCH_PAD4) ? N + N / 4 : N;
 }
 
 // LDS reals a team needs for an N-point transform with exchange layout XCH
 template <int N, int XCH>
 constexpr int lds_reals() {
-  return ((XCH & XCH_CPLX) ? 2 : 1) * ((XCH & XCH_PAD) ? N + N / 32 : N);
+  return ((XCH & XCH_CPLX) ? 2 : 1) * xch_len<N, XCH>();
 }
 
 // One Stockham pass (decimation in time, autosort) on the thread's E values

@@ -418,7 +418,8 @@ __global__ __launch_bounds__(256) void k_fft_test(cx<double> *__restrict__ data,
 
 // ------------------------------------------------------------- benchmark --
 // out-of-place batched transforms for timing the exchange/plan variants:
-// variant bit 0 complex exchange, bit 1 padded exchange, bit 2 radix-4/3 plan
+// variants 0-7: bit 0 complex exchange, bit 1 padded exchange, bit 2 radix-4/3
+// plan; 8-11: complex exchange padded every 16, 8, 4 elements, XOR layout
 template <int N, int S, bool BIG, int XCH>
 __global__ __launch_bounds__(256) void k_fft_bench(const cx<double> *__restrict__ in, cx<double> *__restrict__ out,
                                                    const cx<double> *__restrict__ tw, int batch, int inner) {
@@ -457,6 +458,11 @@ hipError_t launch_bench_n(int variant, const cx<double> *in, cx<double> *out, co
     case 5: return launch_bench_v<N, S, false, 1>(in, out, tw, batch, inner);
     case 6: return launch_bench_v<N, S, false, 2>(in, out, tw, batch, inner);
     case 7: return launch_bench_v<N, S, false, 3>(in, out, tw, batch, inner);
+    // complex exchanges with the timing layouts
+    case 8: return launch_bench_v<N, S, true, fft::XCH_CPLX | fft::XCH_PAD16>(in, out, tw, batch, inner);
+    case 9: return launch_bench_v<N, S, true, fft::XCH_CPLX | fft::XCH_PAD8>(in, out, tw, batch, inner);
+    case 10: return launch_bench_v<N, S, true, fft::XCH_CPLX | fft::XCH_PAD4>(in, out, tw, batch, inner);
+    case 11: return launch_bench_v<N, S, true, fft::XCH_CPLX | fft::XCH_XOR>(in, out, tw, batch, inner);
   }
   return hipErrorInvalidValue;
 }

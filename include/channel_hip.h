@@ -193,9 +193,10 @@ int chan_sync(chan_ctx *c);
 int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout);
 
 /* Diagnostic: batched out-of-place c2c FFT of length n in {384, 768, 1536}
- * with one of the device transform's build variants (bit 0 complex LDS
- * exchange, bit 1 padded exchange, bit 2 radix-4/3 plan; variant >> 3 extra
- * in-register repetitions of the transform per load/store in the timed
+ * with one of the device transform's build variants (variant & 15: 0-7 bit 0
+ * complex LDS exchange, bit 1 padded exchange, bit 2 radix-4/3 plan; 8-11
+ * complex exchange padded every 16, 8, 4 elements or XOR-swizzled; variant >> 4
+ * extra in-register repetitions of the transform per load/store in the timed
  * launches), repeated `reps` times; inout holds the rows on entry and their
  * single transform on return, *ms the average duration of one timed launch. */
 int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int32_t reps, double *inout, double *ms);

@@ -192,7 +192,7 @@ def test_hermitian_symmetry_gpu():
 
 
 @pytest.mark.parametrize("n", [384, 768, 1536])
-@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("variant", range(12))
 def test_fft_build_variants_match_numpy(n, variant):
     say("fft variant", n, variant)
     rng = np.random.default_rng(1000 * n + variant)
