@@ -108,3 +108,23 @@ def test_a_priori_on_dns():
     assert r["loss_last"] < 0.1 * r["loss_first"]
     assert min(r["corr_mlp"]) > 0.7
     assert np.mean(r["corr_mlp"]) > np.mean(r["corr_smagorinsky"]) + 0.3
+
+
+def test_a_posteriori_les():
+    """The closure inside a 31x128x31 LES (body force -div(tau) refreshed
+    every step) started from the filtered 95x128x95 DNS: over 40 steps its
+    distance from the filtered DNS stays below that of the LES without a
+    model (the full study over 100 steps: 0.118 against 0.131)."""
+    from cases import config2
+    from channel_amd import closure
+    from channel_amd.initial import perturbed_laminar
+    print("[case] closure a posteriori, 31x128x31 LES against filtered 95x128x95 DNS", flush=True)
+    cfg = config2(cflmax=0.8, deltat=0.0)
+    V0 = perturbed_laminar(cfg, seed=7, amp=0.08, kmax=8)
+    r = closure.a_posteriori(cfg, V0, horizon=40, check_every=20, models=("none", "mlp"),
+                             log=lambda *a: print(*a, flush=True))
+    none, mlp = r["fluct_err"]["none"], r["fluct_err"]["mlp"]
+    assert len(none) == len(mlp) == 3 and all(np.isfinite(none + mlp))
+    assert 0.01 < none[-1] < 0.5  # the coarse run really departs from the filtered DNS
+    for a, b in zip(mlp[1:], none[1:]):
+        assert a < 0.97 * b
