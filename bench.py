@@ -1,7 +1,7 @@
 """RK3 steps/s of the channel DNS time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
-                    [--config c3] [--no-cpu]
+                    [--config c3] [--no-cpu] [--body-force coriolis]
 
 Workload: the production 384-class grid (nx=191, ny=384, nz=189, a=1.5,
 alfa0=0.5, beta0=1, Re=12431, CPI type 1 with gamma=0.161436, cflmax=1),
@@ -9,6 +9,9 @@ seeded perturbed-laminar start, state resident in HBM.  Times exactly K RK3
 steps after W warm-up steps, bracketed by device synchronisation, and prints
 one JSON line.  Per-kernel durations come from HIP events that the library
 records on its own stream around every launch during the timed steps.
+--body-force coriolis adds the reference's Coriolis body force with the
+defaults of body_forces/coriolis/coriolis.in (2 Ro = 0.02, every mode and
+row), formed from the field at every buildrhs.
 """
 import argparse
 import json
@@ -103,6 +106,7 @@ def main():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel events")
+    ap.add_argument("--body-force", default="none", choices=["none", "coriolis"])
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
                     help="only run the library's plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
@@ -126,6 +130,9 @@ def main():
     V = perturbed_laminar(cfg)
     ch = Channel(cfg, prec)
     ch.set_field(V)
+    if args.body_force == "coriolis":
+        from channel_amd import body_force
+        ch.set_body_force_feedback(*body_force.coriolis(cfg, 0.02))
     ch.initial()
     ch.outstats()
     for _ in range(args.warmup):
@@ -180,7 +187,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == lib.CHAN_FP64 else "f32+f64",
         "data": "synthetic: seeded perturbed-laminar field (splitmix64 seed 20261202)",
-        "config": {"workload": wname, "precision": args.precision, "nxd": ch.nxd, "nzd": ch.nzd},
+        "config": {"workload": wname, "precision": args.precision, "nxd": ch.nxd, "nzd": ch.nzd,
+                   "body_force": args.body_force},
         "step_roofline": {"bytes_per_step": step_bytes, "achieved_GBps": step_bytes * sps / 1e9,
                           "frac": step_bytes * sps / HBM_PEAK},
         "roofline": roof,
