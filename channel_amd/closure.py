@@ -21,8 +21,9 @@
   (spectral x/z derivatives on the coarse grid), and the training data uses
   the same operator on the filtered, truncated DNS field, so that training
   and use see the same inputs.
-The reference has no subgrid model; this follows the north star's closure
-row (box filter, a priori dataset from a GPU DNS, BF16 MFMA training).
+The reference has no subgrid model: the closure is an addition beside the
+DNS step (top-hat filtered GPU DNS data, BF16 MFMA training, a priori and a
+posteriori evaluation).
 """
 import ctypes
 import dataclasses
