@@ -113,6 +113,13 @@ void twiddles(std::vector<cx<T>> &w, int n) {
   w.resize((size_t)fft::tw_table_size(n));
   fft::build_twiddles(w.data(), n);
 }
+// the x pass's table: 768-point rows use the four-step transform
+template <typename T>
+void twiddles_x(std::vector<cx<T>> &w, int n) {
+  if (n != 768) return twiddles(w, n);
+  w.resize((size_t)fft::FS_TW);
+  fft::build_fs768_twiddles(w.data());
+}
 
 }  // namespace
 
@@ -341,7 +348,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
     std::vector<decltype(proto)> tw;
     twiddles(tw, g.nzd);
     if (hipMemcpy(c->twz, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return 1;
-    twiddles(tw, 2 * g.nxd);
+    twiddles_x(tw, 2 * g.nxd);
     if (hipMemcpy(c->twx, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice) != hipSuccess) return 1;
     return 0;
   };
@@ -827,6 +834,39 @@ int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout) {
   hipFree(d);
   if (t) hipFree(t);
   if (e != hipSuccess) return fail((int)e, "fft selftest: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int chan_fft768_fs_test(int32_t sign, int32_t batch, int32_t inner, int32_t reps, double *inout, double *ms) {
+  if (!inout || !ms || batch < 1 || reps < 1 || inner < 1 || inner > 8 || (sign != 1 && sign != -1))
+    return fail(CHAN_E_ARG, "bad argument");
+  cx<double> tw[fft::FS_TW];
+  fft::build_fs768_twiddles(tw);
+  const size_t bytes = sizeof(cx<double>) * 768 * (size_t)batch;
+  cx<double> *a = nullptr, *b = nullptr, *t = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  hipError_t e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMalloc(&t, sizeof tw);
+  if (e == hipSuccess) e = hipMemcpy(a, inout, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(t, tw, sizeof tw, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_fs768_test(a, b, t, batch, sign, 1);
+  if (e == hipSuccess) e = hipMemcpy(inout, b, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_fs768_test(a, b, t, batch, sign, inner);
+  if (e == hipSuccess) e = hipEventRecord(e1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ft = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ft, e0, e1);
+  *ms = ft / reps;
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(a);
+  if (b) hipFree(b);
+  if (t) hipFree(t);
+  if (e != hipSuccess) return fail((int)e, "chan_fft768_fs_test: %s", hipGetErrorString(e));
   return 0;
 }
 

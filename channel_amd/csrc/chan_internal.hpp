@@ -136,6 +136,7 @@ template <typename R>
 hipError_t launch_zfwd(int NZ, hipStream_t st, const cx<R> *B, cx<R> *P, const cx<R> *tw, const Geom &g, int plane0,
                        int nplanes);
 hipError_t launch_fft_test(int N, int sign, hipStream_t st, cx<double> *data, const cx<double> *tw, int batch);
+hipError_t launch_fs768_test(const cx<double> *in, cx<double> *out, const cx<double> *tw, int batch, int sign, int inner);
 hipError_t launch_fft_bench(int N, int sign, int variant, const cx<double> *in, cx<double> *out, const cx<double> *tw,
                             int batch, int inner);
 template <typename R>

@@ -343,5 +343,184 @@ __device__ __forceinline__ void fft_slots(cx<R> (&e)[Plan<N>::E], R *lds, const 
     for (int q = 0; q < RL; q++) e[m + q * (E / RL)] = v[m * RL + q];
 }
 
+// ------------------------------------------------------ four-step 768 --
+// A 768-point transform held by one wavefront, 12 values per lane, computed
+// as 12 x 64: a 12-point DFT in registers, twiddles w768^(n1 t), and a
+// 64-point DFT across the lanes split 4 x 4 x 4 with lane t = p + 16 g,
+// p = d + 4 c.  The radix-4 over the four 16-lane rows g runs after a 4x4
+// transpose of register quads by the permlane swaps (no LDS); the radix-4
+// passes over c and over d each take one LDS exchange, in layouts without
+// bank conflicts for b128 accesses (rows of FS_RG complex values per 16-lane
+// row; c*52 + xi*4 + d, then c*12 + d*49 + eta).  Two exchanges per
+// transform instead of the three of the Stockham plan.
+//
+// fs768_bwd (sign +1) takes the slot layout (slot s of lane t = element
+// t + 64 s) and leaves the samples in the layout P of fs768_index; fs768_fwd
+// (sign -1) takes layout P and returns the slot layout.  Pointwise work
+// between the two (the products) needs no reordering.
+//
+// tw: the FS_TW-entry table of build_fs768_twiddles, forward sign:
+// exp(-2 pi i t/768) for t < 64, exp(-2 pi i p/64) for p < 16,
+// exp(-2 pi i d/16) for d < 4.
+constexpr int FS_RG = 208, FS_LDS = 4 * FS_RG, FS_TW = 84;
+
+template <typename T>
+inline void build_fs768_twiddles(cx<T> *w) {
+  const long double pi2 = 2.0L * 3.141592653589793238462643383279502884L;
+  for (int t = 0; t < 64; t++) w[t] = cx<T>{(T)cosl(-pi2 * t / 768), (T)sinl(-pi2 * t / 768)};
+  for (int q = 0; q < 16; q++) w[64 + q] = cx<T>{(T)cosl(-pi2 * q / 64), (T)sinl(-pi2 * q / 64)};
+  for (int q = 0; q < 4; q++) w[80 + q] = cx<T>{(T)cosl(-pi2 * q / 16), (T)sinl(-pi2 * q / 16)};
+}
+
+// element index n held by lane t in slot 4j+f after fs768_bwd
+__host__ __device__ __forceinline__ int fs768_index(int t, int slot) {
+  const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2, j = slot >> 2, f = slot & 3;
+  const int eta = 3 * d + j, i = eta >> 2, e = eta & 3;
+  const int xi = 3 * c + i, q = xi >> 2, a = xi & 3;
+  return (4 * q + g) + 12 * (a + 4 * (e + 4 * f));
+}
+
+template <typename R>
+__device__ __forceinline__ void fs_swap(cx<R> &a, cx<R> &b, bool rows32) {
+  constexpr int ND = (int)(sizeof(cx<R>) / 4);
+  unsigned ua[ND], ub[ND];
+  __builtin_memcpy(ua, &a, sizeof a);
+  __builtin_memcpy(ub, &b, sizeof b);
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    if (rows32) {
+      const auto r = __builtin_amdgcn_permlane32_swap(ua[k], ub[k], false, false);
+      ua[k] = r[0];
+      ub[k] = r[1];
+    } else {
+      const auto r = __builtin_amdgcn_permlane16_swap(ua[k], ub[k], false, false);
+      ua[k] = r[0];
+      ub[k] = r[1];
+    }
+  }
+  __builtin_memcpy(&a, ua, sizeof a);
+  __builtin_memcpy(&b, ub, sizeof b);
+}
+
+// transpose of each register quad {4q..4q+3} with the four 16-lane rows:
+// afterwards register 4q+r of row g holds what register 4q+g of row r held
+// (an involution)
+template <typename R>
+__device__ __forceinline__ void fs_xpose(cx<R> (&e)[12]) {
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    fs_swap(e[4 * q], e[4 * q + 2], true);
+    fs_swap(e[4 * q + 1], e[4 * q + 3], true);
+    fs_swap(e[4 * q], e[4 * q + 1], false);
+    fs_swap(e[4 * q + 2], e[4 * q + 3], false);
+  }
+}
+
+// v[k] *= w^k, k = 1..3
+template <typename R>
+__device__ __forceinline__ void fs_pow4(cx<R> *v, cx<R> w) {
+  const cx<R> w2 = mul(w, w);
+  v[1] = mul(v[1], w);
+  v[2] = mul(v[2], w2);
+  v[3] = mul(v[3], mul(w2, w));
+}
+// v[k] *= w^k, k = 1..11, powers by a shallow product tree
+template <typename R>
+__device__ __forceinline__ void fs_pow12(cx<R> *v, cx<R> w) {
+  const cx<R> w2 = mul(w, w), w3 = mul(w2, w), w4 = mul(w2, w2), w6 = mul(w3, w3), w8 = mul(w4, w4);
+  v[1] = mul(v[1], w);
+  v[2] = mul(v[2], w2);
+  v[3] = mul(v[3], w3);
+  v[4] = mul(v[4], w4);
+  v[5] = mul(v[5], mul(w4, w));
+  v[6] = mul(v[6], w6);
+  v[7] = mul(v[7], mul(w6, w));
+  v[8] = mul(v[8], w8);
+  v[9] = mul(v[9], mul(w8, w));
+  v[10] = mul(v[10], mul(w6, w4));
+  v[11] = mul(v[11], mul(w8, w3));
+}
+
+template <typename R>
+__device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const cx<R> *tw, int t) {
+  const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
+  cx<R> *row = lds + g * FS_RG;
+  dft<12, +1>(e);
+  fs_pow12(e, conj(tw[t]));
+  fs_xpose(e);
+  const cx<R> w64 = conj(tw[64 + p]);
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    dft<4, +1>(&e[4 * q]);
+    fs_pow4(&e[4 * q], w64);
+  }
+#pragma unroll
+  for (int xi = 0; xi < 12; xi++) row[c * 52 + xi * 4 + d] = e[xi];
+  team_sync<64>();
+  cx<R> v[12];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int cp = 0; cp < 4; cp++) v[4 * i + cp] = row[cp * 52 + (3 * c + i) * 4 + d];
+  team_sync<64>();
+  const cx<R> w16 = conj(tw[80 + d]);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    dft<4, +1>(&v[4 * i]);
+    fs_pow4(&v[4 * i], w16);
+  }
+#pragma unroll
+  for (int eta = 0; eta < 12; eta++) row[c * 12 + d * 49 + eta] = v[eta];
+  team_sync<64>();
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int dp = 0; dp < 4; dp++) e[4 * j + dp] = row[c * 12 + dp * 49 + 3 * d + j];
+  team_sync<64>();
+#pragma unroll
+  for (int j = 0; j < 3; j++) dft<4, +1>(&e[4 * j]);
+}
+
+// the transpose of fs768_bwd's steps with conjugate twiddles
+template <typename R>
+__device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const cx<R> *tw, int t) {
+  const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
+  cx<R> *row = lds + g * FS_RG;
+#pragma unroll
+  for (int j = 0; j < 3; j++) dft<4, -1>(&e[4 * j]);
+#pragma unroll
+  for (int j = 0; j < 3; j++)
+#pragma unroll
+    for (int dp = 0; dp < 4; dp++) row[c * 12 + dp * 49 + 3 * d + j] = e[4 * j + dp];
+  team_sync<64>();
+  cx<R> v[12];
+#pragma unroll
+  for (int eta = 0; eta < 12; eta++) v[eta] = row[c * 12 + d * 49 + eta];
+  team_sync<64>();
+  const cx<R> w16 = tw[80 + d];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    fs_pow4(&v[4 * i], w16);
+    dft<4, -1>(&v[4 * i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int cp = 0; cp < 4; cp++) row[cp * 52 + (3 * c + i) * 4 + d] = v[4 * i + cp];
+  team_sync<64>();
+#pragma unroll
+  for (int xi = 0; xi < 12; xi++) e[xi] = row[c * 52 + xi * 4 + d];
+  team_sync<64>();
+  const cx<R> w64 = tw[64 + p];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    fs_pow4(&e[4 * q], w64);
+    dft<4, -1>(&e[4 * q]);
+  }
+  fs_xpose(e);
+  fs_pow12(e, tw[t]);
+  dft<12, -1>(e);
+}
+
 }  // namespace fft
 }  // namespace chan

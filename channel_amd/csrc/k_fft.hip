@@ -172,20 +172,38 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   using PL = fft::Plan<M>;
   constexpr int T = PL::T, E = PL::E, NT = ZB * T;
   constexpr int MO = xmodes<E>();
-  __shared__ __align__(16) R fl[ZB * fft::lds_reals<M, XXCH>()];
+  // 768-point rows use the four-step transform (fs768_bwd/fwd); the samples
+  // between the two sit in its permuted layout, which the pointwise CFL and
+  // products do not see
+  constexpr bool FS = M == 768;
+  constexpr int TEAM_R = FS ? 2 * fft::FS_LDS : fft::lds_reals<M, XXCH>();
+  __shared__ __align__(16) R fl[ZB * TEAM_R];
   __shared__ double red[NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
   const int tid = threadIdx.x, zi = tid / T, t = tid % T;
   const int zb = blockIdx.x % NZB, pl = blockIdx.x / NZB;
   const int iy = plane0 + pl - 1;
-  R *fr = fl + zi * fft::lds_reals<M, XXCH>();
+  R *fr = fl + zi * TEAM_R;
+  cx<R> *frc = reinterpret_cast<cx<R> *>(fr);
 
   // twiddles in LDS: their reads then wait on the LDS counter, not behind
   // the row loads still in flight on the vector-memory counter
-  constexpr int NTW = fft::plan_tw_size(M, XBIG);
+  constexpr int NTW = FS ? fft::FS_TW : fft::plan_tw_size(M, XBIG);
   __shared__ __align__(16) cx<R> twl[NTW > 0 ? NTW : 1];
   for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
   __syncthreads();
+  auto bwd = [&](cx<R>(&v)[E]) {
+    if constexpr (FS)
+      fft::fs768_bwd(v, frc, twl, t);
+    else
+      fft::fft_slots<M, +1, XBIG, XXCH>(v, fr, twl, t);
+  };
+  auto fwd = [&](cx<R>(&v)[E]) {
+    if constexpr (FS)
+      fft::fs768_fwd(v, frc, twl, t);
+    else
+      fft::fft_slots<M, -1, XBIG, XXCH>(v, fr, twl, t);
+  };
 
   // Hermitian extension of row zi to M points: slot s holds mode k = t+T*s
   // (k <= nx) or the conjugate of mode M-k (k >= M-nx); Im of mode 0 is
@@ -271,7 +289,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
 #pragma unroll
     for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
     if constexpr (!SHFL) load_raw(2, f);  // in flight during the first transform
-    fft::fft_slots<M, +1, XBIG, XXCH>(e, fr, twl, t);
+    bwd(e);
 #pragma unroll
     for (int s = 0; s < E; s++) {
       u[s] = e[s].x;
@@ -281,7 +299,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
       extend(rw, f);
     else
       fix_ext(f);
-    fft::fft_slots<M, +1, XBIG, XXCH>(f, fr, twl, t);
+    bwd(f);
 #pragma unroll
     for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
@@ -321,7 +339,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
         }
         e[s] = cx<R>{a, b};
       }
-      fft::fft_slots<M, -1, XBIG, XXCH>(e, fr, twl, t);
+      fwd(e);
       // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
       cx<R> mz[MO];
       mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
@@ -414,6 +432,53 @@ __global__ __launch_bounds__(256) void k_fft_test(cx<double> *__restrict__ data,
   if (act)
 #pragma unroll
     for (int s = 0; s < E; s++) x[t + T * s] = e[s];
+}
+
+// ------------------------------------------------------ four-step test --
+// Rows of 768 complex FP64 values in natural order.  sign +1: slot layout ->
+// fs768_bwd -> written back in natural order through fs768_index; sign -1:
+// read in layout P -> fs768_fwd -> natural order.  inner > 1 first runs
+// inner-1 backward/forward pairs (for timing; the data grows by 768 per pair).
+__global__ __launch_bounds__(256) void k_fs768_test(const cx<double> *__restrict__ in, cx<double> *__restrict__ out,
+                                                    const cx<double> *__restrict__ tw, int batch, int sign, int inner) {
+  __shared__ __align__(16) cx<double> lds[4 * fft::FS_LDS];
+  __shared__ cx<double> twl[fft::FS_TW];
+  for (int i = threadIdx.x; i < fft::FS_TW; i += 256) twl[i] = tw[i];
+  __syncthreads();
+  const int team = threadIdx.x / 64, t = threadIdx.x % 64;
+  const int row = blockIdx.x * 4 + team;
+  const bool act = row < batch;
+  const cx<double> *x = in + (size_t)(act ? row : 0) * 768;
+  cx<double> *L = lds + team * fft::FS_LDS;
+  cx<double> e[12];
+  if (sign > 0) {
+#pragma unroll
+    for (int s = 0; s < 12; s++) e[s] = x[t + 64 * s];
+    for (int k = 1; k < inner; k++) {
+      fft::fs768_bwd(e, L, twl, t);
+      fft::fs768_fwd(e, L, twl, t);
+    }
+    fft::fs768_bwd(e, L, twl, t);
+    if (act)
+#pragma unroll
+      for (int s = 0; s < 12; s++) out[(size_t)row * 768 + fft::fs768_index(t, s)] = e[s];
+  } else {
+#pragma unroll
+    for (int s = 0; s < 12; s++) e[s] = x[fft::fs768_index(t, s)];
+    for (int k = 1; k < inner; k++) {
+      fft::fs768_fwd(e, L, twl, t);
+      fft::fs768_bwd(e, L, twl, t);
+    }
+    fft::fs768_fwd(e, L, twl, t);
+    if (act)
+#pragma unroll
+      for (int s = 0; s < 12; s++) out[(size_t)row * 768 + t + 64 * s] = e[s];
+  }
+}
+
+hipError_t launch_fs768_test(const cx<double> *in, cx<double> *out, const cx<double> *tw, int batch, int sign, int inner) {
+  hipLaunchKernelGGL(k_fs768_test, dim3((unsigned)((batch + 3) / 4)), dim3(256), 0, 0, in, out, tw, batch, sign, inner);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------- benchmark --

@@ -85,6 +85,7 @@ SIGNATURES = {
     "chan_fft_selftest": (ctypes.c_int, [_I32, _I32, _I32, _D]),
     "chan_copy_selftest": (ctypes.c_int, [ctypes.c_int64, _I32, _D]),
     "chan_fft_bench": (ctypes.c_int, [_I32, _I32, _I32, _I32, _I32, _D, _D]),
+    "chan_fft768_fs_test": (ctypes.c_int, [_I32, _I32, _I32, _I32, _D, _D]),
     # include/channel_closure.h
     "chan_mlp_create": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
     "chan_mlp_destroy": (ctypes.c_int, [_P]),
@@ -96,6 +97,17 @@ SIGNATURES = {
     "chan_mlp_train": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _F, _D]),
     "chan_mlp_bench_forward": (ctypes.c_int, [_P, _I32, _D]),
 }
+
+
+def fft768_fs(x, sign, inner=1, reps=1):
+    """The x pass's four-step 768-point transform of the rows of x (natural
+    order in and out); returns (result, ms per timed launch)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    b, n = x.shape
+    assert n == 768
+    ms = ctypes.c_double()
+    check(load().chan_fft768_fs_test(int(sign), b, int(inner), int(reps), dptr(x), ctypes.byref(ms)))
+    return x, ms.value
 
 
 def fft_bench(x, sign, variant, reps=10):

@@ -201,6 +201,15 @@ int chan_fft_selftest(int32_t n, int32_t sign, int32_t batch, double *inout);
  * single transform on return, *ms the average duration of one timed launch. */
 int chan_fft_bench(int32_t n, int32_t sign, int32_t batch, int32_t variant, int32_t reps, double *inout, double *ms);
 
+/* Diagnostic: the four-step 768-point transform of the x pass (one
+ * wavefront per row; 12 values per lane, permlane-swap transpose and two LDS
+ * exchanges) on `batch` rows of 768 complex128 values in natural order:
+ * sign +1 the unnormalised backward transform, -1 the forward one.  inout
+ * holds the rows on entry and their transforms on return; *ms the average
+ * duration of `reps` timed launches, each of which first runs inner-1
+ * backward/forward pairs (1 <= inner <= 8). */
+int chan_fft768_fs_test(int32_t sign, int32_t batch, int32_t inner, int32_t reps, double *inout, double *ms);
+
 /* Diagnostic: a plain device-to-device copy of `bytes` (multiple of 32) done
  * `reps` times by the library's own streaming copy kernel; *ms receives the
  * average duration.  Used to calibrate hardware byte counters. */

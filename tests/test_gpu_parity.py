@@ -171,15 +171,16 @@ def test_orr_sommerfeld_gpu():
     assert rel_l2(g.get_field(), o.get_field()) < 1e-12
 
 
-@pytest.mark.parametrize("nx,nz", [(511, 7), (7, 511)])
-def test_1536_point_transform_paths(nx, nz):
-    """Grids whose x or z transform is 1536 points long (2nxd or nzd = 1536):
-    128-lane transform teams, the x pass's LDS-built Hermitian extension and
-    mirror split, two teams per z block -- the paths the 511x512x511 grid
-    runs.  Every mode carries a small random part, so the extension and
-    truncation at the highest modes are exercised."""
+@pytest.mark.parametrize("nx,nz", [(255, 7), (511, 7), (7, 511)])
+def test_wide_transform_paths(nx, nz):
+    """Grids with long x or z transforms: 2nxd = 768 (the four-step x-pass
+    transform of the 191x384x189 grid), 2nxd = 1536 and nzd = 1536 (128-lane
+    teams, the x pass's LDS-built Hermitian extension and mirror split, two
+    teams per z block -- the paths of the 511x512x511 grid).  Every mode
+    carries a small random part, so the extension and truncation at the
+    highest modes are exercised."""
     cfg = tiny(nx=nx, ny=16, nz=nz)
-    say("1536-point paths", nx, nz)
+    say("wide transform paths", nx, nz)
     V = seeded_field(cfg)
     rng = np.random.default_rng(nx + nz)
     R = 1e-4 * (rng.standard_normal(V.shape) + 1j * rng.standard_normal(V.shape))
