@@ -212,6 +212,19 @@ def test_hermitian_symmetry_gpu():
     assert err < 1e-14 * np.max(np.abs(V[:, :, 0, :]))
 
 
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_four_step_768_matches_numpy(sign):
+    """The x pass's four-step 768-point transform (permlane transpose, two
+    LDS exchanges, permuted layout between backward and forward) in natural
+    order against numpy."""
+    say("four-step 768", sign)
+    rng = np.random.default_rng(768 + sign)
+    x = rng.standard_normal((37, 768)) + 1j * rng.standard_normal((37, 768))
+    y, _ = lib.fft768_fs(x, sign)
+    ref = np.fft.fft(x, axis=1) if sign < 0 else np.fft.ifft(x, axis=1) * 768
+    assert rel_l2(y, ref) < 1e-15
+
+
 @pytest.mark.parametrize("n", [384, 768, 1536])
 @pytest.mark.parametrize("variant", range(12))
 def test_fft_build_variants_match_numpy(n, variant):
