@@ -416,43 +416,65 @@ __device__ __forceinline__ void fs_xpose(cx<R> (&e)[12]) {
   }
 }
 
-// v[k] *= w^k, k = 1..3
+// a * conj(b)
 template <typename R>
-__device__ __forceinline__ void fs_pow4(cx<R> *v, cx<R> w) {
-  const cx<R> w2 = mul(w, w);
-  v[1] = mul(v[1], w);
-  v[2] = mul(v[2], w2);
-  v[3] = mul(v[3], mul(w2, w));
+__device__ __forceinline__ cx<R> mulc(cx<R> a, cx<R> b) {
+  return cx<R>{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
 }
-// v[k] *= w^k, k = 1..11, powers by a shallow product tree
+
+// The twiddle powers of one lane, forward sign, shared by every transform of
+// the lane's rows: w768^(n1 t) for n1 = 1..11, w64^(a p) and w16^(e d) for
+// a, e = 1..3 (powers by a shallow product tree from the table entries).
 template <typename R>
-__device__ __forceinline__ void fs_pow12(cx<R> *v, cx<R> w) {
+struct FsTw {
+  cx<R> w768[11], w64[3], w16[3];
+};
+
+template <typename R>
+__device__ __forceinline__ FsTw<R> fs_twiddles(const cx<R> *tw, int t) {
+  FsTw<R> z;
+  const cx<R> w = tw[t];
   const cx<R> w2 = mul(w, w), w3 = mul(w2, w), w4 = mul(w2, w2), w6 = mul(w3, w3), w8 = mul(w4, w4);
-  v[1] = mul(v[1], w);
-  v[2] = mul(v[2], w2);
-  v[3] = mul(v[3], w3);
-  v[4] = mul(v[4], w4);
-  v[5] = mul(v[5], mul(w4, w));
-  v[6] = mul(v[6], w6);
-  v[7] = mul(v[7], mul(w6, w));
-  v[8] = mul(v[8], w8);
-  v[9] = mul(v[9], mul(w8, w));
-  v[10] = mul(v[10], mul(w6, w4));
-  v[11] = mul(v[11], mul(w8, w3));
+  z.w768[0] = w;
+  z.w768[1] = w2;
+  z.w768[2] = w3;
+  z.w768[3] = w4;
+  z.w768[4] = mul(w4, w);
+  z.w768[5] = w6;
+  z.w768[6] = mul(w6, w);
+  z.w768[7] = w8;
+  z.w768[8] = mul(w8, w);
+  z.w768[9] = mul(w6, w4);
+  z.w768[10] = mul(w8, w3);
+  const cx<R> u = tw[64 + (t & 15)], u2 = mul(u, u);
+  z.w64[0] = u;
+  z.w64[1] = u2;
+  z.w64[2] = mul(u2, u);
+  const cx<R> v = tw[80 + (t & 3)], v2 = mul(v, v);
+  z.w16[0] = v;
+  z.w16[1] = v2;
+  z.w16[2] = mul(v2, v);
+  return z;
+}
+
+// v[k] *= w[k-1] (forward) or conj(w[k-1]) (backward), k = 1..n-1
+template <int S, int n, typename R>
+__device__ __forceinline__ void fs_apply(cx<R> *v, const cx<R> *w) {
+#pragma unroll
+  for (int k = 1; k < n; k++) v[k] = S < 0 ? mul(v[k], w[k - 1]) : mulc(v[k], w[k - 1]);
 }
 
 template <typename R>
-__device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const cx<R> *tw, int t) {
+__device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const FsTw<R> &w, int t) {
   const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
   cx<R> *row = lds + g * FS_RG;
   dft<12, +1>(e);
-  fs_pow12(e, conj(tw[t]));
+  fs_apply<+1, 12>(e, w.w768);
   fs_xpose(e);
-  const cx<R> w64 = conj(tw[64 + p]);
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     dft<4, +1>(&e[4 * q]);
-    fs_pow4(&e[4 * q], w64);
+    fs_apply<+1, 4>(&e[4 * q], w.w64);
   }
 #pragma unroll
   for (int xi = 0; xi < 12; xi++) row[c * 52 + xi * 4 + d] = e[xi];
@@ -463,11 +485,10 @@ __device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const cx<R
 #pragma unroll
     for (int cp = 0; cp < 4; cp++) v[4 * i + cp] = row[cp * 52 + (3 * c + i) * 4 + d];
   team_sync<64>();
-  const cx<R> w16 = conj(tw[80 + d]);
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     dft<4, +1>(&v[4 * i]);
-    fs_pow4(&v[4 * i], w16);
+    fs_apply<+1, 4>(&v[4 * i], w.w16);
   }
 #pragma unroll
   for (int eta = 0; eta < 12; eta++) row[c * 12 + d * 49 + eta] = v[eta];
@@ -483,7 +504,7 @@ __device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const cx<R
 
 // the transpose of fs768_bwd's steps with conjugate twiddles
 template <typename R>
-__device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const cx<R> *tw, int t) {
+__device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const FsTw<R> &w, int t) {
   const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
   cx<R> *row = lds + g * FS_RG;
 #pragma unroll
@@ -497,10 +518,9 @@ __device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const cx<R
 #pragma unroll
   for (int eta = 0; eta < 12; eta++) v[eta] = row[c * 12 + d * 49 + eta];
   team_sync<64>();
-  const cx<R> w16 = tw[80 + d];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    fs_pow4(&v[4 * i], w16);
+    fs_apply<-1, 4>(&v[4 * i], w.w16);
     dft<4, -1>(&v[4 * i]);
   }
 #pragma unroll
@@ -511,14 +531,13 @@ __device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const cx<R
 #pragma unroll
   for (int xi = 0; xi < 12; xi++) e[xi] = row[c * 52 + xi * 4 + d];
   team_sync<64>();
-  const cx<R> w64 = tw[64 + p];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    fs_pow4(&e[4 * q], w64);
+    fs_apply<-1, 4>(&e[4 * q], w.w64);
     dft<4, -1>(&e[4 * q]);
   }
   fs_xpose(e);
-  fs_pow12(e, tw[t]);
+  fs_apply<-1, 12>(e, w.w768);
   dft<12, -1>(e);
 }
 

@@ -192,15 +192,18 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   __shared__ __align__(16) cx<R> twl[NTW > 0 ? NTW : 1];
   for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
   __syncthreads();
+  // the four-step transform's twiddle powers, shared by the row's five transforms
+  fft::FsTw<R> fw;
+  if constexpr (FS) fw = fft::fs_twiddles(twl, t);
   auto bwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
-      fft::fs768_bwd(v, frc, twl, t);
+      fft::fs768_bwd(v, frc, fw, t);
     else
       fft::fft_slots<M, +1, XBIG, XXCH>(v, fr, twl, t);
   };
   auto fwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
-      fft::fs768_fwd(v, frc, twl, t);
+      fft::fs768_fwd(v, frc, fw, t);
     else
       fft::fft_slots<M, -1, XBIG, XXCH>(v, fr, twl, t);
   };
@@ -450,15 +453,16 @@ __global__ __launch_bounds__(256) void k_fs768_test(const cx<double> *__restrict
   const bool act = row < batch;
   const cx<double> *x = in + (size_t)(act ? row : 0) * 768;
   cx<double> *L = lds + team * fft::FS_LDS;
+  const fft::FsTw<double> fw = fft::fs_twiddles(twl, t);
   cx<double> e[12];
   if (sign > 0) {
 #pragma unroll
     for (int s = 0; s < 12; s++) e[s] = x[t + 64 * s];
     for (int k = 1; k < inner; k++) {
-      fft::fs768_bwd(e, L, twl, t);
-      fft::fs768_fwd(e, L, twl, t);
+      fft::fs768_bwd(e, L, fw, t);
+      fft::fs768_fwd(e, L, fw, t);
     }
-    fft::fs768_bwd(e, L, twl, t);
+    fft::fs768_bwd(e, L, fw, t);
     if (act)
 #pragma unroll
       for (int s = 0; s < 12; s++) out[(size_t)row * 768 + fft::fs768_index(t, s)] = e[s];
@@ -466,10 +470,10 @@ __global__ __launch_bounds__(256) void k_fs768_test(const cx<double> *__restrict
 #pragma unroll
     for (int s = 0; s < 12; s++) e[s] = x[fft::fs768_index(t, s)];
     for (int k = 1; k < inner; k++) {
-      fft::fs768_fwd(e, L, twl, t);
-      fft::fs768_bwd(e, L, twl, t);
+      fft::fs768_fwd(e, L, fw, t);
+      fft::fs768_bwd(e, L, fw, t);
     }
-    fft::fs768_fwd(e, L, twl, t);
+    fft::fs768_fwd(e, L, fw, t);
     if (act)
 #pragma unroll
       for (int s = 0; s < 12; s++) out[(size_t)row * 768 + t + 64 * s] = e[s];
