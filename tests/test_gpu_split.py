@@ -94,3 +94,19 @@ def test_split_laminar_fixed_point():
     g = split_channel(cfg, V0)
     g.steps(10)
     assert np.max(np.abs(g.get_field() - V0)) < 1e-8
+
+
+def test_split_four_step_x_transform_vs_oracle():
+    """The FP32 four-step 768-point x transform (2nxd = 768, as at
+    191x384x189) in the split step against the FP64 oracle."""
+    cfg = tiny(nx=255, ny=16, nz=7)
+    say("split, 2nxd = 768, 3 steps")
+    V = seeded_field(cfg)
+    g = split_channel(cfg, V)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    for _ in range(3):
+        rows_close(g.step(), o.step(), SPLIT_TOL)
+    e = rel_l2(g.get_field(), o.get_field())
+    print("split rel L2 after 3 steps:", e)
+    assert 1e-14 < e <= SPLIT_TOL
