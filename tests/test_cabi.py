@@ -88,3 +88,14 @@ def test_unknown_precision_mode_is_rejected():
     rc = lib.load().chan_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == -1 and not h.value
     assert b"precision" in lib.load().chan_last_error()
+
+
+def test_wrong_cpi_type_is_the_reference_stop():
+    """CPI with CPI_type outside {0, 1} stops the reference ("Wrong selection
+    of CPI_Type", linsolve_blocking.inc:93-95); chan_create refuses it."""
+    import ctypes
+    p = lib.params_from(tiny(cpi=True, cpi_type=2))
+    h = ctypes.c_void_p()
+    rc = lib.load().chan_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == lib.CHAN_E_CPITYPE and not h.value
+    assert b"CPI_Type" in lib.load().chan_last_error()
