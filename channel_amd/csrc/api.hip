@@ -155,6 +155,10 @@ struct chan_ctx {
   cx<double> *F0 = nullptr;
   double *gain = nullptr;
   int src[3] = {-1, -1, -1};
+  // the force of the last buildrhs, kept on request: [3][Ny][ps]; kept_state
+  // 0 = nothing kept yet, 1 = Fkeep holds it, 2 = that buildrhs had no force
+  cx<double> *Fkeep = nullptr;
+  int kept_state = 0;
   DevScalars *sc = nullptr;
   // host scalars
   double time = 0, deltat = 0;
@@ -380,7 +384,7 @@ int chan_destroy(chan_ctx *c) {
   }
   for (auto e : c->pool) hipEventDestroy(e);
   void *bufs[] = {c->V, c->old, c->rhs, c->L, c->col00, c->A, c->B, c->P, c->twz, c->twx,
-                  c->der, c->d0fac, c->y, c->dy, c->sc, c->stat, c->F0, c->gain};
+                  c->der, c->d0fac, c->y, c->dy, c->sc, c->stat, c->F0, c->gain, c->Fkeep};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (c->st) hipStreamDestroy(c->st);
@@ -512,6 +516,36 @@ int chan_get_body_force(chan_ctx *c, double *F) {
   return 0;
 }
 
+int chan_keep_body_force(chan_ctx *c, int32_t on) {
+  if (!c) return fail(CHAN_E_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->kept_state = 0;
+  if (!on) {
+    if (c->Fkeep) HIPCHK(hipFree(c->Fkeep));
+    c->Fkeep = nullptr;
+    return 0;
+  }
+  int rc;
+  if (!c->Fkeep && (rc = dalloc(&c->Fkeep, 3 * (size_t)c->g.Ny * c->g.ps))) return rc;
+  return 0;
+}
+
+int chan_get_applied_body_force(chan_ctx *c, double *F) {
+  if (!c || !F) return fail(CHAN_E_ARG, "null argument");
+  if (!c->Fkeep) return fail(CHAN_E_STATE, "get_applied_body_force without keep_body_force");
+  if (!c->kept_state) return fail(CHAN_E_STATE, "get_applied_body_force before a buildrhs");
+  const size_t n = (size_t)3 * c->g.npair * c->g.Ny;
+  if (c->kept_state == 2) {
+    std::memset(F, 0, n * sizeof(cx<double>));
+    return 0;
+  }
+  cx<double> *stage = static_cast<cx<double> *>(c->B);
+  HIPCHK(launch_to_host_layout(c->st, c->Fkeep, stage, c->g));
+  HIPCHK(hipMemcpyAsync(F, stage, n * sizeof(cx<double>), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
 int chan_get_rhs(chan_ctx *c, double *R) {
   if (!c || !R) return fail(CHAN_E_ARG, "null argument");
   // rhs is [2][ny-1][ps]: the same transpose as the field with ny-1 rows
@@ -601,7 +635,7 @@ static int products(chan_ctx *c, int plane0, int nplanes, bool prods, int cfl) {
 }
 
 template <typename R>
-static int rhs_t(chan_ctx *c, const double ode[3]) {
+static int rhs_t(chan_ctx *c, const double ode[3], bool keep) {
   RhsArgs<R> a;
   a.P = static_cast<const cx<R> *>(c->P);
   a.V = c->V;
@@ -617,23 +651,35 @@ static int rhs_t(chan_ctx *c, const double ode[3]) {
   a.ni = c->prm.ni;
   a.chunk = 48;
   a.force = c->F0 || c->gain;
+  if (keep) c->kept_state = a.force ? 1 : 2;
   if (a.force) {
     ForceArgs f = force_args(c);
+    int rc;
+    if (keep) {
+      f.out = c->Fkeep;
+      if ((rc = timed(c, "force_keep", [&] { return launch_force(c->st, f); }))) return rc;
+    }
     f.out = c->rhs;
-    int rc = timed(c, "force", [&] { return launch_force_terms(c->st, f); });
-    if (rc) return rc;
+    if ((rc = timed(c, "force", [&] { return launch_force_terms(c->st, f); }))) return rc;
   }
   return timed(c, "rhs", [&] { return launch_rhs<R>(c->st, a); });
+}
+
+// keep: store the applied force when keeping is on (chan_step asks only in
+// its last substep, whose force is the one that stays)
+static int buildrhs(chan_ctx *c, const double ode[3], bool compute_cfl, bool keep) {
+  const Geom &g = c->g;
+  int rc;
+  if ((rc = products(c, 0, g.Ny, true, compute_cfl ? 1 : 0))) return rc;
+  keep = keep && c->Fkeep;
+  return c->split ? rhs_t<float>(c, ode, keep) : rhs_t<double>(c, ode, keep);
 }
 }  // extern "C++"
 
 int chan_buildrhs(chan_ctx *c, const double ode[3], int32_t compute_cfl) {
   if (!c || !ode) return fail(CHAN_E_ARG, "null argument");
   if (!c->field_set) return fail(CHAN_E_STATE, "buildrhs before set_field");
-  const Geom &g = c->g;
-  int rc;
-  if ((rc = products(c, 0, g.Ny, true, compute_cfl ? 1 : 0))) return rc;
-  return c->split ? rhs_t<float>(c, ode) : rhs_t<double>(c, ode);
+  return buildrhs(c, ode, compute_cfl != 0, true);
 }
 
 int chan_linsolve(chan_ctx *c, double lambda) {
@@ -762,7 +808,8 @@ int chan_step(chan_ctx *c, double row[11]) {
   const double *ode[3] = {RK1, RK2, RK3};
   for (int k = 0; k < 3; k++) {
     c->time = c->time + 2.0 / ode[k][0] * c->deltat;
-    if ((rc = chan_buildrhs(c, ode[k], k == 2))) return rc;
+    if (!c->field_set) return fail(CHAN_E_STATE, "step before set_field");
+    if ((rc = buildrhs(c, ode[k], k == 2, k == 2))) return rc;
     if ((rc = chan_linsolve(c, ode[k][0] / c->deltat))) return rc;
   }
   return chan_outstats(c, row);

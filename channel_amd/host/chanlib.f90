@@ -93,6 +93,22 @@ MODULE chanlib
       type(C_PTR), value :: ctx
       real(C_DOUBLE), intent(out) :: row(11)
     END FUNCTION
+    integer(C_INT) FUNCTION chan_set_body_force_feedback(ctx, src, gain) bind(C, name="chan_set_body_force_feedback")
+      import :: C_INT, C_INT32_T, C_PTR, C_DOUBLE
+      type(C_PTR), value :: ctx
+      integer(C_INT32_T), intent(in) :: src(3)
+      real(C_DOUBLE), intent(in) :: gain(*)
+    END FUNCTION
+    integer(C_INT) FUNCTION chan_keep_body_force(ctx, on) bind(C, name="chan_keep_body_force")
+      import :: C_INT, C_INT32_T, C_PTR
+      type(C_PTR), value :: ctx
+      integer(C_INT32_T), value :: on
+    END FUNCTION
+    integer(C_INT) FUNCTION chan_get_applied_body_force(ctx, F) bind(C, name="chan_get_applied_body_force")
+      import :: C_INT, C_PTR, C_DOUBLE_COMPLEX
+      type(C_PTR), value :: ctx
+      complex(C_DOUBLE_COMPLEX), intent(out) :: F(*)
+    END FUNCTION
     integer(C_INT) FUNCTION chan_sync(ctx) bind(C, name="chan_sync")
       import :: C_INT, C_PTR
       type(C_PTR), value :: ctx

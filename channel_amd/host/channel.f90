@@ -10,81 +10,14 @@
 ! same restart semantics (time_from_restart, get_record: channel.f90:52-71,
 ! dnsdata.f90:181-218) and the same fail-stop behaviour.
 !
-MODULE host_io
-  USE, intrinsic :: iso_c_binding
-  USE chanlib
-  IMPLICIT NONE
-  integer(C_INT) :: nx, ny, nz, npy, CPI_type
-  real(C_DOUBLE) :: alfa0, beta0, ni, a, ymin, ymax, deltat, cflmax, time, dt_field, dt_save, t_max, gamma
-  real(C_DOUBLE) :: u0, uN, meanpx, meanpz, meanflowx, meanflowz
-  logical :: CPI, time_from_restart
-  integer(C_SIZE_T) :: nstep
-CONTAINS
-
-  SUBROUTINE read_dnsin()
-    OPEN(15, file='dns.in', status='old', action='read')
-    READ(15, *) nx, ny, nz
-    READ(15, *) alfa0, beta0
-    READ(15, *) ni
-    ni = 1/ni
-    READ(15, *) a, ymin, ymax
-    READ(15, *) CPI, CPI_type, gamma
-    READ(15, *) meanpx, meanpz
-    READ(15, *) meanflowx, meanflowz
-    READ(15, *) u0, uN
-    READ(15, *) deltat, cflmax, time
-    READ(15, *) dt_field, dt_save, t_max, time_from_restart
-    READ(15, *) nstep
-    READ(15, *) npy
-    CLOSE(15)
-  END SUBROUTINE read_dnsin
-
-  ! restart file: 3 int32 + 7 float64 header, then V(-1:ny+1,-nz:nz,0:nx,1:3)
-  SUBROUTINE read_restart(fname, V, found)
-    character(len=*), intent(in) :: fname
-    complex(C_DOUBLE_COMPLEX), intent(inout) :: V(-1:ny+1,-nz:nz,0:nx,1:3)
-    logical, intent(out) :: found
-    integer(C_INT) :: r_nx, r_ny, r_nz, io
-    real(C_DOUBLE) :: r_alfa0, r_beta0, r_ni, r_a, r_ymin, r_ymax, r_time
-    OPEN(UNIT=100, FILE=TRIM(fname), access="stream", status="old", action="read", iostat=io)
-    found = (io == 0)
-    IF (.NOT. found) RETURN
-    WRITE(*,*) "Reading from file "//TRIM(fname)
-    READ(100, POS=1) r_nx, r_ny, r_nz, r_alfa0, r_beta0, r_ni, r_a, r_ymin, r_ymax, r_time
-    IF (r_nx /= nx .OR. r_ny /= ny .OR. r_nz /= nz .OR. r_alfa0 /= alfa0 .OR. r_beta0 /= beta0 .OR. &
-        r_ni /= ni .OR. r_a /= a .OR. r_ymin /= ymin .OR. r_ymax /= ymax) THEN
-      WRITE(*,*) "ERROR: mismatch in metadata between restart file and dns.in. Stopping."
-      WRITE(*,*) "From .out file:"
-      WRITE(*,*) r_nx, r_ny, r_nz, r_alfa0, r_beta0, r_ni, r_a, r_ymin, r_ymax
-      WRITE(*,*) "From dns.in:"
-      WRITE(*,*) nx, ny, nz, alfa0, beta0, ni, a, ymin, ymax
-      STOP
-    END IF
-    READ(100, POS=69) V
-    CLOSE(100)
-    time = r_time
-  END SUBROUTINE read_restart
-
-  SUBROUTINE save_restart(fname, V, t)
-    character(len=*), intent(in) :: fname
-    complex(C_DOUBLE_COMPLEX), intent(in) :: V(-1:ny+1,-nz:nz,0:nx,1:3)
-    real(C_DOUBLE), intent(in) :: t
-    OPEN(UNIT=100, FILE=TRIM(fname), access="stream", status="replace", action="write")
-    WRITE(100, POS=1) nx, ny, nz, alfa0, beta0, ni, a, ymin, ymax, t
-    WRITE(100, POS=69) V
-    CLOSE(100)
-  END SUBROUTINE save_restart
-
-  SUBROUTINE wanna_continue()
-    character :: yn
-    WRITE(*,"(a,$)") "Do you want to continue? (y/n) "
-    READ(*,*) yn
-    IF (yn == 'n') STOP
-  END SUBROUTINE wanna_continue
-
-END MODULE host_io
-
-
+! Two environment variables stand in for the reference's compile-time
+! choices:
+!   CHANNEL_BODYFORCE = am_f1 | am_butterfly | coriolis  -- the module that
+!     header.h's bodyforce / BODYFORCE_MODULES select (header.h:28-31):
+!     config_body_force, set_body_force before every buildrhs, and a
+!     Force.cart.N.out beside each Dati.cart.N.out (dnsdata.f90:904-906);
+!   CHANNEL_PRECISION = fp64 (default) | split  -- the FP32/FP64 split step.
+!
 PROGRAM channel
   USE, intrinsic :: iso_c_binding
   USE chanlib
@@ -96,6 +29,9 @@ PROGRAM channel
   complex(C_DOUBLE_COMPLEX), allocatable, target :: V(:,:,:,:)
   real(C_DOUBLE), allocatable, target :: y(:)
   real(C_DOUBLE) :: row(11), deltat_from_dnsin, time0, selectime, curr_dt, cols(9)
+  real(C_DOUBLE), allocatable, target :: gain(:,:,:)
+  complex(C_DOUBLE_COMPLEX), allocatable :: F(:,:,:,:)
+  integer(C_INT32_T) :: src(3)
   integer(C_INT32_T) :: sizes(5)
   integer(C_SIZE_T) :: istep, ifield
   integer :: io, iy
@@ -104,12 +40,15 @@ PROGRAM channel
   character(len=40) :: istring
 
   CALL read_dnsin()
+  CALL read_env()
   deltat_from_dnsin = deltat
   prm = params()
   CALL chan_check(chan_create(prm, ctx), "chan_create")
   CALL chan_check(chan_sizes(ctx, sizes), "chan_sizes")
   ALLOCATE(V(-1:ny+1,-nz:nz,0:nx,1:3), y(-1:ny+1))
   V = 0
+  CALL chan_check(chan_setup_host(prm, sizes, c_loc(y), C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, C_NULL_PTR), &
+                  "chan_setup_host")
 
   ! Runtimedata (init_memory, dnsdata.f90:159-176)
   INQUIRE(FILE="Runtimedata", EXIST=rtd_exists)
@@ -129,8 +68,6 @@ PROGRAM channel
   CALL read_restart("Dati.cart.out", V, found)
   IF (.NOT. found) THEN
     WRITE(*,*) "Generating initial field..."
-    CALL chan_check(chan_setup_host(prm, sizes, c_loc(y), C_NULL_PTR, C_NULL_PTR, C_NULL_PTR, C_NULL_PTR), &
-                    "chan_setup_host")
     DO iy = -1, ny+1
       V(iy,0,0,1) = 3*0.5*y(iy)*(2-y(iy))
     END DO
@@ -167,7 +104,14 @@ PROGRAM channel
   WRITE(*,*) "!====================================================!"
   WRITE(*,"(A,I5,A,I5,A,I5)") "   nx =", nx, "   ny =", ny, "   nz =", nz
   WRITE(*,"(A,I5,A,I5)") "   nxd =", sizes(4), "  nzd =", sizes(5)
+  IF (precision_sel == CHAN_SPLIT) WRITE(*,"(A)") "   FP32/FP64 split step"
   WRITE(*,*) " "
+  IF (LEN_TRIM(bf_name) > 0) THEN
+    ALLOCATE(gain(0:2*nz+1, -1:ny+1, 0:2), F(-1:ny+1,-nz:nz,0:nx,1:3))
+    CALL body_force_table(y, src, gain)
+    CALL chan_check(chan_set_body_force_feedback(ctx, src, gain), "chan_set_body_force_feedback")
+    CALL chan_check(chan_keep_body_force(ctx, 1), "chan_keep_body_force")
+  END IF
 
   CALL chan_check(chan_set_field(ctx, V), "chan_set_field")
   CALL chan_check(chan_get_scalars(ctx, sc), "chan_get_scalars")
@@ -200,7 +144,7 @@ CONTAINS
 
   FUNCTION params() result(p)
     type(chan_params) :: p
-    p%nx = nx; p%ny = ny; p%nz = nz; p%precision = CHAN_FP64
+    p%nx = nx; p%ny = ny; p%nz = nz; p%precision = precision_sel
     p%alfa0 = alfa0; p%beta0 = beta0; p%ni = ni; p%a = a; p%ymin = ymin; p%ymax = ymax
     p%cpi = MERGE(1, 0, CPI); p%cpi_type = CPI_type; p%gamma = gamma
     p%meanpx = meanpx; p%meanpz = meanpz; p%meanflowx = meanflowx; p%meanflowz = meanflowz
@@ -234,6 +178,13 @@ CONTAINS
         WRITE(*,*) "Writing "//TRIM(fn)//" at time ", time
         CALL chan_check(chan_get_field(ctx, V), "chan_get_field")
         CALL save_restart(fn, V, time)
+        IF (LEN_TRIM(bf_name) > 0) THEN
+          ! the force of the last substep, as F holds it after a step
+          fn = "Force.cart."//TRIM(ADJUSTL(istring))//".out"
+          WRITE(*,*) "Writing "//TRIM(fn)//" at time ", time
+          CALL chan_check(chan_get_applied_body_force(ctx, F), "chan_get_applied_body_force")
+          CALL save_restart(fn, F, time)
+        END IF
         prev_was_close = .FALSE.
       ELSE
         prev_was_close = .TRUE.

@@ -76,6 +76,8 @@ SIGNATURES = {
     "chan_set_body_force": (ctypes.c_int, [_P, _D]),
     "chan_set_body_force_feedback": (ctypes.c_int, [_P, ctypes.POINTER(_I32), _D]),
     "chan_get_body_force": (ctypes.c_int, [_P, _D]),
+    "chan_keep_body_force": (ctypes.c_int, [_P, _I32]),
+    "chan_get_applied_body_force": (ctypes.c_int, [_P, _D]),
     "chan_step": (ctypes.c_int, [_P, _D]),
     "chan_steps": (ctypes.c_int, [_P, _I32, _D]),
     "chan_timing": (ctypes.c_int, [_P, _I32]),

@@ -143,6 +143,17 @@ class Channel:
         lib.check(self._L.chan_get_body_force(self.h, lib.dptr(F)))
         return F
 
+    def keep_body_force(self, on=True):
+        """Keep a copy of the force each step applies (its third substep's)."""
+        lib.check(self._L.chan_keep_body_force(self.h, 1 if on else 0))
+
+    def get_applied_body_force(self):
+        """The force the last step applied, as the reference's F holds it
+        after a time step (Force.cart.N.out); needs keep_body_force()."""
+        F = np.zeros(self.shape, dtype=np.complex128, order="F")
+        lib.check(self._L.chan_get_applied_body_force(self.h, lib.dptr(F)))
+        return F
+
     def steps(self, n):
         r = np.zeros((n, 11))
         if n:

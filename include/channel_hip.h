@@ -173,10 +173,23 @@ int chan_stats(chan_ctx *c, const double *avg, double *out);
  * chan_get_body_force: the force that the next buildrhs applies to the
  * resident field, wall and ghost rows included, in the layout of V (zero
  * when no force is set); the payload of the reference's Force.cart.N.out
- * (dnsdata.f90:904-906) for a static force. */
+ * (dnsdata.f90:904-906) for a static force.
+ *
+ * chan_keep_body_force(c, 1): from then on every buildrhs that applies a
+ * force also stores it (chan_step stores only its third substep's, the one
+ * that stays); chan_keep_body_force(c, 0) stops and frees the copy (one
+ * field, 3*(ny+3)*(2nz+1)*(nx+1) complex FP64).  chan_get_applied_body_force
+ * returns that copy in the layout of V: the reference's F after a time step
+ * -- set_body_force applied to the field of the third substep, then the wall
+ * rows of buildrhs (dnsdata.f90:616-629) -- which is what Force.cart.N.out
+ * holds (dnsdata.f90:904-906) when a module sets F from V.  Zero when that
+ * buildrhs applied no force; CHAN_E_STATE when keeping is off or no buildrhs
+ * has run since it was turned on. */
 int chan_set_body_force(chan_ctx *c, const double *F);
 int chan_set_body_force_feedback(chan_ctx *c, const int32_t src[3], const double *gain);
 int chan_get_body_force(chan_ctx *c, double *F);
+int chan_keep_body_force(chan_ctx *c, int32_t on);
+int chan_get_applied_body_force(chan_ctx *c, double *F);
 
 /* Per-kernel timing with HIP events recorded on the library's stream around
  * every launch (off by default).  chan_kernel_times fills up to cap entries of

@@ -171,6 +171,52 @@ def test_force_removal_and_bad_source():
     assert e.value.code == lib.CHAN_E_ARG
 
 
+def test_applied_force_is_the_third_substeps():
+    """chan_keep_body_force / chan_get_applied_body_force: after each step the
+    kept force equals the oracle's F, which its last buildrhs set from the
+    field of the third substep and then gave the wall rows (what
+    Force.cart.N.out holds, dnsdata.f90:904-906); the force of the resident
+    field is a different one."""
+    cfg = config1()
+    say("applied force: keep, coriolis feedback, static force")
+    V = seeded_field(cfg)
+    g, o = channel(cfg, V), oracle(cfg, V)
+    for keep in (False, True):
+        with pytest.raises(lib.ChanError) as e:
+            g.get_applied_body_force()  # off, then on with no buildrhs yet
+        assert e.value.code == lib.CHAN_E_STATE
+        if keep:
+            g.keep_body_force()
+    g.step()
+    o.step()
+    assert np.all(g.get_applied_body_force() == 0)  # no force in that step
+    par, table = MODULES["coriolis"]
+    g.set_body_force_feedback(*table(cfg))
+    o.set_body_force_module("coriolis", *par)
+    for k in range(3):
+        g.step()
+        o.step()
+        Fa, Fo = g.get_applied_body_force(), o.get_body_force()
+        e = rel_l2(Fa, Fo)
+        print("step", k + 1, "applied force rel L2", e)
+        assert e <= TOL
+    assert rel_l2(g.get_body_force(), Fa) > 1e-8
+    assert rel_l2(g.get_field(), o.get_field()) <= TOL
+    # a static force: F after the step is F0 with the wall rows
+    g.set_body_force_feedback(None, None)
+    o.set_body_force_module(None)
+    F = smooth_force(cfg, 4)
+    g.set_body_force(F)
+    o.set_body_force(F)
+    g.step()
+    o.step()
+    assert rel_l2(g.get_applied_body_force(), o.get_body_force()) < 1e-14
+    g.keep_body_force(False)
+    with pytest.raises(lib.ChanError) as e:
+        g.get_applied_body_force()
+    assert e.value.code == lib.CHAN_E_STATE
+
+
 def test_split_mode_with_feedback():
     cfg = config1()
     say("split mode with coriolis feedback, 10 steps")

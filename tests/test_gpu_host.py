@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from cases import config1, rel_l2, seeded_field
+from channel_amd import body_force as bf
 from channel_amd import dns
 from oracle import oracle as orc
 
@@ -124,3 +125,87 @@ def test_host_restart_continues_run(tmp_path):
     V, t, _ = dns.read_restart(str(tmp_path / "Dati.cart.out"), cfg)
     assert t == pytest.approx(8 * dt, rel=1e-14)
     assert rel_l2(V, o.get_field()) <= 1e-12
+
+
+CORIOLIS_IN = "0.5     ! 2*Ro\n9.0     ! kz_cutoff\n0.4     ! y_threshold\n"
+HOST_MODULES = {  # CHANNEL_BODYFORCE -> the oracle's module parameters
+    "am_f1": lambda d: (1000.0, 500.0),  # am_pardec.inc:1
+    "am_butterfly": lambda d: (1000.0, 500.0),
+    "coriolis": lambda d: tuple(bf.read_coriolis_in(str(d / "coriolis.in")).values()),
+}
+
+
+@pytest.mark.parametrize("module,precision", [("coriolis", "fp64"), ("am_f1", "fp64"), ("am_butterfly", "split")])
+def test_host_body_force_module(tmp_path, module, precision):
+    """CHANNEL_BODYFORCE selects a module as header.h's bodyforce does
+    (config_body_force, set_body_force before every buildrhs) and each
+    Dati.cart.N.out gets its Force.cart.N.out (dnsdata.f90:904-906);
+    CHANNEL_PRECISION=split runs the split step.  Runtimedata, the snapshots
+    and the force files against the oracle with the same module."""
+    cfg = config1()
+    cfg.deltat = 1e-3  # amp = 1000 is explicit in the RK3 substeps
+    nsteps = 6
+    cfg.t_max = nsteps * cfg.deltat
+    cfg.nstep = 1000
+    cfg.dt_field = 2 * cfg.deltat
+    cfg.dt_save = 0.0
+    cfg.time_from_restart = False
+    V0 = seeded_field(cfg)
+    cfg.write(str(tmp_path / "dns.in"))
+    (tmp_path / "coriolis.in").write_text(CORIOLIS_IN)
+    dns.save_restart(str(tmp_path / "Dati.cart.out"), cfg, V0, 0.0)
+    print("[case] host", module, precision, nsteps, "steps", flush=True)
+    env = dict(os.environ, CHANNEL_BODYFORCE=module, CHANNEL_PRECISION=precision)
+    r = subprocess.run([EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=120,
+                       stdin=subprocess.DEVNULL, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Using bodyforce" in r.stdout
+
+    o = orc.Oracle(cfg)
+    o.set_field(V0)
+    o.set_scalars(deltat=cfg.deltat, time=0.0)
+    o.set_body_force_module(module, *HOST_MODULES[module](tmp_path))
+    o.initial()
+    rows_o = [o.outstats()]
+    fields, forces = {}, {}
+    for k in range(nsteps):
+        rows_o.append(o.step())
+        fields[k + 1], forces[k + 1] = o.get_field(), o.get_body_force()
+    rows_o = np.array(rows_o)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    gmax = {"am_f1": 1000.0, "am_butterfly": 1000.0, "coriolis": HOST_MODULES["coriolis"](tmp_path)[0]}
+
+    rows_h = dns.parse_runtimedata(str(tmp_path / "Runtimedata"))
+    assert rows_h.shape == rows_o.shape, (rows_h.shape, rows_o.shape)
+    np.testing.assert_array_less(np.abs(rows_h - rows_o), 10 * tol * np.maximum(np.abs(rows_o), 1.0))
+    V, _, _ = dns.read_restart(str(tmp_path / "Dati.cart.out"), cfg)
+    assert rel_l2(V, fields[nsteps]) <= tol
+    # the force changes the run
+    assert rel_l2(fields[nsteps], _unforced(cfg, V0, nsteps)) > 1e3 * tol
+
+    snaps = sorted(p for p in os.listdir(tmp_path) if p.startswith("Force.cart."))
+    assert len(snaps) == nsteps // 2, os.listdir(tmp_path)
+    for name in snaps:
+        Fh, ts, _ = dns.read_restart(str(tmp_path / name), cfg)
+        Vs, ts2, _ = dns.read_restart(str(tmp_path / name.replace("Force", "Dati")), cfg)
+        k = int(round(ts / cfg.deltat))
+        assert ts == ts2
+        assert rel_l2(Vs, fields[k]) <= tol, (name, k)
+        # F = g V on a mask: its error is the field's error times |g|; the
+        # masked modes alone can be far smaller than the field (am_f1 damps
+        # them at rate amp), so the bar is taken relative to |g| ||V||
+        Fo = forces[k]
+        err = np.linalg.norm(Fh - Fo) / (gmax[module] * np.linalg.norm(fields[k]))
+        print(name, "force error / (|g| ||V||)", err, "plain rel L2", rel_l2(Fh, Fo))
+        assert err <= tol, (name, k, err)
+        assert np.linalg.norm(Fo) > 0
+
+
+def _unforced(cfg, V0, n):
+    o = orc.Oracle(cfg)
+    o.set_field(V0)
+    o.set_scalars(deltat=cfg.deltat, time=0.0)
+    o.initial()
+    o.steps(n)
+    return o.get_field()
+
