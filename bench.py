@@ -1,7 +1,7 @@
 """RK3 steps/s of the channel DNS time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
-                    [--config c3] [--no-cpu] [--body-force coriolis]
+                    [--config c3] [--no-cpu] [--no-split] [--body-force coriolis]
 
 Workload: the production 384-class grid (nx=191, ny=384, nz=189, a=1.5,
 alfa0=0.5, beta0=1, Re=12431, CPI type 1 with gamma=0.161436, cflmax=1),
@@ -12,6 +12,9 @@ records on its own stream around every launch during the timed steps.
 --body-force coriolis adds the reference's Coriolis body force with the
 defaults of body_forces/coriolis/coriolis.in (2 Ro = 0.02, every mode and
 row), formed from the field at every buildrhs.
+With --precision fp64 (the default) the line also carries a "split" object:
+the same workload timed the same way with the FP32/FP64 split step, and its
+rate over the FP64 one (--no-split leaves it out).
 """
 import argparse
 import json
@@ -105,6 +108,7 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"])
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="no split-step leg beside the FP64 line")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel events")
     ap.add_argument("--body-force", default="none", choices=["none", "coriolis"])
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
@@ -128,27 +132,33 @@ def main():
     wname, cfg = workload(args.config)
     prec = lib.CHAN_FP64 if args.precision == "fp64" else lib.CHAN_SPLIT
     V = perturbed_laminar(cfg)
-    ch = Channel(cfg, prec)
-    ch.set_field(V)
-    if args.body_force == "coriolis":
-        from channel_amd import body_force
-        ch.set_body_force_feedback(*body_force.coriolis(cfg, 0.02))
-    ch.initial()
-    ch.outstats()
-    for _ in range(args.warmup):
-        ch.step()
-    if not args.no_timing:
-        ch.timing(True)
-    ch.sync()
-    t0 = time.perf_counter()
-    rows = ch.steps(args.steps)
-    ch.sync()
-    t1 = time.perf_counter()
-    ktimes = ch.kernel_times() if not args.no_timing else {}
-    ch.timing(False)
-    elapsed = t1 - t0
-    if not np.all(np.isfinite(rows)):
-        raise SystemExit("non-finite Runtimedata in the timed steps")
+
+    def timed_run(prec):
+        """W warm-up steps, then K timed steps between device synchronisations;
+        returns (elapsed s, per-kernel times, the context, still open)."""
+        ch = Channel(cfg, prec)
+        ch.set_field(V)
+        if args.body_force == "coriolis":
+            from channel_amd import body_force
+            ch.set_body_force_feedback(*body_force.coriolis(cfg, 0.02))
+        ch.initial()
+        ch.outstats()
+        for _ in range(args.warmup):
+            ch.step()
+        if not args.no_timing:
+            ch.timing(True)
+        ch.sync()
+        t0 = time.perf_counter()
+        rows = ch.steps(args.steps)
+        ch.sync()
+        t1 = time.perf_counter()
+        ktimes = ch.kernel_times() if not args.no_timing else {}
+        ch.timing(False)
+        if not np.all(np.isfinite(rows)):
+            raise SystemExit("non-finite Runtimedata in the timed steps")
+        return t1 - t0, ktimes, ch
+
+    elapsed, ktimes, ch = timed_run(prec)
     sps = args.steps / elapsed
 
     per, step_bytes = algorithmic_bytes(cfg, ch.nxd, ch.nzd, 16 if prec == lib.CHAN_FP64 else 8)
@@ -195,6 +205,15 @@ def main():
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in sorted(ktimes.items())},
     }
     ch.close()
+    if prec == lib.CHAN_FP64 and not args.no_split:
+        el2, kt2, ch2 = timed_run(lib.CHAN_SPLIT)
+        ch2.close()
+        sps2 = args.steps / el2
+        _, split_bytes = algorithmic_bytes(cfg, ch.nxd, ch.nzd, 8)
+        out["split"] = {"value": sps2, "unit": "steps/s", "ms_per_step": 1e3 * el2 / args.steps,
+                        "ratio_to_fp64": sps2 / sps, "dtype": "f32+f64",
+                        "step_roofline": {"bytes_per_step": split_bytes, "frac": split_bytes * sps2 / HBM_PEAK},
+                        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in sorted(kt2.items())}}
     if not args.no_cpu and args.gpus == 1:
         out["cpu_baseline"] = cpu_baseline(cfg, V)
     else:
