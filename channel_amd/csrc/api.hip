@@ -113,9 +113,15 @@ void twiddles(std::vector<cx<T>> &w, int n) {
   w.resize((size_t)fft::tw_table_size(n));
   fft::build_twiddles(w.data(), n);
 }
-// the x pass's table: 768-point rows use the four-step transform
+// the x pass's table: 768-point rows use the four-step transform, 1536-point
+// rows its two-wave form
 template <typename T>
 void twiddles_x(std::vector<cx<T>> &w, int n) {
+  if (n == 1536) {
+    w.resize((size_t)fft::FS2_TW);
+    fft::build_fs1536_twiddles(w.data());
+    return;
+  }
   if (n != 768) return twiddles(w, n);
   w.resize((size_t)fft::FS_TW);
   fft::build_fs768_twiddles(w.data());

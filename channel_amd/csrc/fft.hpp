@@ -541,5 +541,99 @@ __device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const FsTw
   dft<12, -1>(e);
 }
 
+// ---------------------------------------------------- two-wave 1536 --
+// A 1536-point transform held by a 128-lane team of two wavefronts, 12 values
+// per lane: one radix-2 step between the waves, then the four-step 768-point
+// transform in each wave.  area: the team's LDS, two rows of FS_LDS complex
+// values; wave h = t / 64 runs its fs768 passes in row h.
+//
+// fs1536_bwd (sign +1) takes the slot layout (slot s of lane t = element
+// t + 128 s).  Splitting the output into even and odd bins, the pairs
+// (n, n + 768) sit in slots s and s + 6 of one lane:
+//   a_n = x_n + x_{n+768},  b_n = (x_n - x_{n+768}) w1536^n,
+// and X_{2m} / X_{2m+1} are the 768-point transforms of a / b.  Each wave
+// sends the six values that the other one needs and takes six, so that wave
+// 0 holds a and wave 1 holds b in the fs768 slot layout (element n' =
+// t%64 + 64 s'); afterwards wave h holds X_{2m+h} for m in the layout P of
+// fs768_index.
+//
+// fs1536_fwd (sign -1) is the transpose: wave h's fs768_fwd gives the
+// transform E (h = 0) or O (h = 1) of the samples 2m + h, which it stores in
+// its row in natural order; fs1536_mode then forms Z_k = E_k + w1536^k O_k and
+// Z_{1536-k} = E_{768-k} + conj(w1536^k) O_{768-k} (w1536^768 = -1).  A
+// team_sync after the last fs1536_mode frees the rows.
+//
+// tw: the FS2_TW-entry table of build_fs1536_twiddles: build_fs768_twiddles'
+// FS_TW entries, then exp(-2 pi i t/1536) for t < 64 and exp(-2 pi i j/24)
+// for j < 12.
+constexpr int FS2_TW = FS_TW + 64 + 12;
+
+template <typename T>
+inline void build_fs1536_twiddles(cx<T> *w) {
+  build_fs768_twiddles(w);
+  const long double pi2 = 2.0L * 3.141592653589793238462643383279502884L;
+  for (int t = 0; t < 64; t++) w[FS_TW + t] = cx<T>{(T)cosl(-pi2 * t / 1536), (T)sinl(-pi2 * t / 1536)};
+  for (int j = 0; j < 12; j++) w[FS_TW + 64 + j] = cx<T>{(T)cosl(-pi2 * j / 24), (T)sinl(-pi2 * j / 24)};
+}
+
+// w1536^(t + 64 j), forward sign, t < 64, j < 12
+template <typename R>
+__device__ __forceinline__ cx<R> fs2_twiddle(const cx<R> *tw, int t, int j) {
+  return mul(tw[FS_TW + t], tw[FS_TW + 64 + j]);
+}
+
+// c ? x : y per component (a select of whole complex values can become a
+// two-entry private array indexed by c)
+template <typename R>
+__device__ __forceinline__ cx<R> pick(bool c, cx<R> x, cx<R> y) {
+  return cx<R>{c ? x.x : y.x, c ? x.y : y.y};
+}
+
+template <typename R>
+__device__ __forceinline__ void fs1536_bwd(cx<R> (&e)[12], cx<R> *area, const FsTw<R> &w, const cx<R> *tw, int t) {
+  const int h = t >> 6, tl = t & 63;
+  cx<R> *own = area + h * FS_LDS;
+  const cx<R> *other = area + (1 - h) * FS_LDS;
+  cx<R> keep[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const cx<R> a = add(e[s], e[s + 6]);
+    const cx<R> b = mulc(sub(e[s], e[s + 6]), fs2_twiddle(tw, tl, h + 2 * s));  // n = tl + 64 (h + 2 s)
+    keep[s] = pick(h != 0, b, a);
+    own[s * 64 + tl] = pick(h != 0, a, b);
+  }
+  team_sync<128>();
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    const cx<R> r = other[s * 64 + tl];
+    e[2 * s] = pick(h != 0, r, keep[s]);
+    e[2 * s + 1] = pick(h != 0, keep[s], r);
+  }
+  team_sync<128>();
+  fs768_bwd(e, own, w, tl);
+}
+
+template <typename R>
+__device__ __forceinline__ void fs1536_fwd(cx<R> (&e)[12], cx<R> *area, const FsTw<R> &w, int t) {
+  const int h = t >> 6, tl = t & 63;
+  cx<R> *own = area + h * FS_LDS;
+  fs768_fwd(e, own, w, tl);
+#pragma unroll
+  for (int s = 0; s < 12; s++) own[tl + 64 * s] = e[s];
+  team_sync<128>();
+}
+
+// Z_k and Z_{1536-k} for 0 <= k < 768 from the rows that fs1536_fwd filled
+// (k = 0 gives Z_0 for both)
+template <typename R>
+__device__ __forceinline__ void fs1536_mode(const cx<R> *area, const cx<R> *tw, int k, cx<R> &zk, cx<R> &zm) {
+  const cx<R> *E = area, *O = area + FS_LDS;
+  const cx<R> wk = fs2_twiddle(tw, k & 63, k >> 6);
+  zk = add(E[k], mul(O[k], wk));
+  const int km = k ? 768 - k : 0;
+  const cx<R> zr = add(E[km], mulc(O[km], wk));
+  zm = pick(k != 0, zr, zk);
+}
+
 }  // namespace fft
 }  // namespace chan

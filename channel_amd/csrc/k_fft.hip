@@ -172,11 +172,13 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   using PL = fft::Plan<M>;
   constexpr int T = PL::T, E = PL::E, NT = ZB * T;
   constexpr int MO = xmodes<E>();
-  // 768-point rows use the four-step transform (fs768_bwd/fwd); the samples
-  // between the two sit in its permuted layout, which the pointwise CFL and
-  // products do not see
-  constexpr bool FS = M == 768;
-  constexpr int TEAM_R = FS ? 2 * fft::FS_LDS : fft::lds_reals<M, XXCH>();
+  // 768-point rows use the four-step transform (fs768_bwd/fwd), 1536-point
+  // rows its two-wave form (fs1536_bwd/fwd); the samples between the two
+  // directions sit in a permuted layout, which the pointwise CFL and products
+  // do not see
+  constexpr bool FS = M == 768, FS2 = M == 1536;
+  constexpr int TEAM_R = FS ? 2 * fft::FS_LDS : (FS2 ? 4 * fft::FS_LDS : fft::lds_reals<M, XXCH>());
+  static_assert(!FS2 || (T == 128 && E == 12 && MO == 4), "the two-wave 1536 transform needs 128-lane teams");
   __shared__ __align__(16) R fl[ZB * TEAM_R];
   __shared__ double red[NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
@@ -188,22 +190,26 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
 
   // twiddles in LDS: their reads then wait on the LDS counter, not behind
   // the row loads still in flight on the vector-memory counter
-  constexpr int NTW = FS ? fft::FS_TW : fft::plan_tw_size(M, XBIG);
+  constexpr int NTW = FS ? fft::FS_TW : (FS2 ? fft::FS2_TW : fft::plan_tw_size(M, XBIG));
   __shared__ __align__(16) cx<R> twl[NTW > 0 ? NTW : 1];
   for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
   __syncthreads();
   // the four-step transform's twiddle powers, shared by the row's five transforms
   fft::FsTw<R> fw;
-  if constexpr (FS) fw = fft::fs_twiddles(twl, t);
+  if constexpr (FS || FS2) fw = fft::fs_twiddles(twl, t & 63);
   auto bwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
       fft::fs768_bwd(v, frc, fw, t);
+    else if constexpr (FS2)
+      fft::fs1536_bwd(v, frc, fw, twl, t);
     else
       fft::fft_slots<M, +1, XBIG, XXCH>(v, fr, twl, t);
   };
   auto fwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
       fft::fs768_fwd(v, frc, fw, t);
+    else if constexpr (FS2)
+      fft::fs1536_fwd(v, frc, fw, t);
     else
       fft::fft_slots<M, -1, XBIG, XXCH>(v, fr, twl, t);
   };
@@ -344,17 +350,32 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
       }
       fwd(e);
       // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
-      cx<R> mz[MO];
-      mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
       cx<R> *da = B + ((size_t)(pl * 6 + qa) * NZB + zb) * Nx * ZB + zi;
       cx<R> *db = B + ((size_t)(pl * 6 + qb) * NZB + zb) * Nx * ZB + zi;
+      auto put = [&](int k, cx<R> z, cx<R> q) {
+        da[(size_t)k * ZB] = cx<R>{(z.x + q.x) * R(0.5), (z.y - q.y) * R(0.5)};
+        db[(size_t)k * ZB] = cx<R>{(z.y + q.y) * R(0.5), (q.x - z.x) * R(0.5)};
+      };
+      if constexpr (FS2) {
+        // wave h forms the modes 256 h .. 256 h + 255 (nx < 512 when M = 1536)
+        const int h = t >> 6, tl = t & 63;
 #pragma unroll
-      for (int s = 0; s < MO; s++) {
-        const int k = t + T * s;
-        if (k <= nx) {
-          const cx<R> z = e[s], q = mz[s];
-          da[(size_t)k * ZB] = cx<R>{(z.x + q.x) * R(0.5), (z.y - q.y) * R(0.5)};
-          db[(size_t)k * ZB] = cx<R>{(z.y + q.y) * R(0.5), (q.x - z.x) * R(0.5)};
+        for (int s = 0; s < MO; s++) {
+          const int k = 256 * h + tl + 64 * s;
+          if (k <= nx) {
+            cx<R> z, q;
+            fft::fs1536_mode(frc, twl, k, z, q);
+            put(k, z, q);
+          }
+        }
+        fft::team_sync<T>();  // the rows are the next transform's exchange area
+      } else {
+        cx<R> mz[MO];
+        mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
+#pragma unroll
+        for (int s = 0; s < MO; s++) {
+          const int k = t + T * s;
+          if (k <= nx) put(k, e[s], mz[s]);
         }
       }
     }

@@ -171,14 +171,15 @@ def test_orr_sommerfeld_gpu():
     assert rel_l2(g.get_field(), o.get_field()) < 1e-12
 
 
-@pytest.mark.parametrize("nx,nz", [(255, 7), (511, 7), (7, 511)])
+@pytest.mark.parametrize("nx,nz", [(255, 7), (511, 7), (383, 7), (7, 511)])
 def test_wide_transform_paths(nx, nz):
     """Grids with long x or z transforms: 2nxd = 768 (the four-step x-pass
-    transform of the 191x384x189 grid), 2nxd = 1536 and nzd = 1536 (128-lane
-    teams, the x pass's LDS-built Hermitian extension and mirror split, two
-    teams per z block -- the paths of the 511x512x511 grid).  Every mode
-    carries a small random part, so the extension and truncation at the
-    highest modes are exercised."""
+    transform of the 191x384x189 grid), 2nxd = 1536 (128-lane teams of two
+    waves: the two-wave four-step transform, with all 512 modes at nx = 511
+    and fewer than it holds at nx = 383) and nzd = 1536 (two teams per z
+    block) -- the paths of the 511x512x511 grid.  Every mode carries a small
+    random part, so the extension and truncation at the highest modes are
+    exercised."""
     cfg = tiny(nx=nx, ny=16, nz=nz)
     say("wide transform paths", nx, nz)
     V = seeded_field(cfg)

@@ -96,11 +96,13 @@ def test_split_laminar_fixed_point():
     assert np.max(np.abs(g.get_field() - V0)) < 1e-8
 
 
-def test_split_four_step_x_transform_vs_oracle():
-    """The FP32 four-step 768-point x transform (2nxd = 768, as at
-    191x384x189) in the split step against the FP64 oracle."""
-    cfg = tiny(nx=255, ny=16, nz=7)
-    say("split, 2nxd = 768, 3 steps")
+@pytest.mark.parametrize("nx", [255, 511])
+def test_split_four_step_x_transform_vs_oracle(nx):
+    """The FP32 four-step x transforms in the split step against the FP64
+    oracle: 768 points (nx = 255, as at 191x384x189) and the two-wave 1536
+    points (nx = 511, as at 511x512x511)."""
+    cfg = tiny(nx=nx, ny=16, nz=7)
+    say("split, nx =", nx, "3 steps")
     V = seeded_field(cfg)
     g = split_channel(cfg, V)
     o = orc.Oracle(cfg)
