@@ -117,49 +117,28 @@ __device__ __forceinline__ void block_max_atomic(double m, double *red, unsigned
   if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
 }
 
-// value of the team-mate holding element M-k of a transform in slot layout:
-// for t > 0 it is lane T-t, slot E-1-s; for t = 0 the thread itself, slot
-// (E-s) mod E.  Teams of up to one wavefront use cross-lane moves, larger
-// teams go through LDS (buf: M reals of the team).
 // x modes per thread of the x pass: nxd >= 3(nx+1)/2 bounds Nx by M/3
 template <int E>
 constexpr int xmodes() {
   return (E + 2) / 3;
 }
 
+// value of the team-mate holding element M-k of a transform in slot layout:
+// for t > 0 it is lane T-t, slot E-1-s; for t = 0 the thread itself, slot
+// (E-s) mod E.  Teams of one wavefront at most (the 128-lane teams of the
+// 1536-point rows combine their modes in fs1536_mode instead).
 template <int T, int E, typename R>
-__device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmodes<E>()], int t, int nmode, R *buf) {
+__device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmodes<E>()], int t, int nmode) {
+  static_assert(T <= 64, "mirror_pairs works within one wavefront");
   constexpr int MO = xmodes<E>();
-  if constexpr (T <= 64) {
-    const int lane = (int)(threadIdx.x & 63);
-    const int src = lane - t + ((T - t) % T);
+  const int lane = (int)(threadIdx.x & 63);
+  const int src = lane - t + ((T - t) % T);
 #pragma unroll
-    for (int s = 0; s < MO; s++) {
-      if (s * T < nmode) {  // uniform across the team
-        const cx<R> send = e[E - 1 - s];
-        const R rx = __shfl(send.x, src, 64), ry = __shfl(send.y, src, 64);
-        m[s] = t ? cx<R>{rx, ry} : e[(E - s) % E];
-      }
-    }
-  } else {
-    constexpr int M = T * E;
-#pragma unroll
-    for (int half = 0; half < 2; half++) {
-#pragma unroll
-      for (int s = 0; s < E; s++) buf[t + T * s] = half ? e[s].y : e[s].x;
-      fft::team_sync<T>();
-#pragma unroll
-      for (int s = 0; s < MO; s++) {
-        const int k = t + T * s;
-        if (k < nmode) {
-          const R v = buf[k == 0 ? 0 : M - k];
-          if (half)
-            m[s].y = v;
-          else
-            m[s].x = v;
-        }
-      }
-      fft::team_sync<T>();
+  for (int s = 0; s < MO; s++) {
+    if (s * T < nmode) {  // uniform across the team
+      const cx<R> send = e[E - 1 - s];
+      const R rx = __shfl(send.x, src, 64), ry = __shfl(send.y, src, 64);
+      m[s] = t ? cx<R>{rx, ry} : e[(E - s) % E];
     }
   }
 }
@@ -371,7 +350,7 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
         fft::team_sync<T>();  // the rows are the next transform's exchange area
       } else {
         cx<R> mz[MO];
-        mirror_pairs<T, E, R>(e, mz, t, nx + 1, fr);
+        mirror_pairs<T, E, R>(e, mz, t, nx + 1);
 #pragma unroll
         for (int s = 0; s < MO; s++) {
           const int k = t + T * s;
