@@ -85,3 +85,35 @@ def test_split_profiles_stay_within_one_percent_of_fp64():
             worst = max(worst, dev(sg[key], sd[key]))
     print("largest profile deviation over 500 steps:", worst)
     assert worst <= 1e-2
+
+
+FORCED = os.path.join(os.path.dirname(__file__), "golden", "forced_run_c1.npz")
+
+
+@pytest.mark.parametrize("precision,bound", [(lib.CHAN_FP64, 1e-8), (lib.CHAN_SPLIT, 1e-4)], ids=["fp64", "split"])
+def test_coriolis_500_steps_vs_oracle_record(precision, bound):
+    """The Coriolis module (2 Ro = 0.5, kz_cutoff = 9, y_threshold = 0.4) as a
+    feedback force over 500 steps against the oracle's record
+    (tests/golden/forced_run_c1.npz, written by make_forced_run.py with the
+    oracle's literal restatement of body_forces/coriolis)."""
+    from channel_amd import body_force as bf
+    from channel_amd.solver import Channel
+    say("coriolis 500 steps at 31x64x31, precision", precision)
+    ref = np.load(FORCED, allow_pickle=False)
+    cfg = config1()
+    g = Channel(cfg, precision)
+    g.set_field(seeded_field(cfg))
+    g.set_body_force_feedback(*bf.coriolis(cfg, 0.5, 9.0, 0.4))
+    rows = g.steps(100)
+    st = {100: g.stats()}
+    rows = np.vstack([rows, g.steps(400)])
+    st[500] = g.stats()
+    r, rr = rows[ref["row_steps"] - 1], ref["rows"]
+    for col in (0, 1, 2, 3, 4, 5, 6, 9, 10):
+        d = dev(r[:, col], rr[:, col])
+        print("column", col, "max rel deviation", d)
+        assert d <= bound, (col, d)
+    for n in (100, 500):
+        d = profile_dev(st[n], ref, n)
+        print("profiles after", n, "steps:", d)
+        assert max(d.values()) <= bound, (n, d)
