@@ -143,13 +143,26 @@ __device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmo
   }
 }
 
+// teams per x-pass block: one per row of a z block, except for 1536-point
+// rows, whose 128-lane teams hold 29 KB of LDS each in FP64.  Their blocks
+// take half a z block (two rows), so that two blocks share a CU and one
+// block's barriers overlap the other's work (25% less time per launch than
+// four-row blocks at 511x512x511).  The halves read 32 of each 64 contiguous
+// bytes of the z-block interleave; they are numbered 8 blocks apart, so that
+// round-robin dispatch over the 8 XCDs puts both on one XCD and its L2.
+template <int M>
+constexpr int xteams() {
+  return M == 1536 ? 2 : ZB;
+}
 template <int M, typename R, bool PRODUCTS>
-__global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
+__global__ __launch_bounds__(xteams<M>() * fft::Plan<M>::T, 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
-                                                                double dz, R factor, unsigned long long *__restrict__ cfl_bits) {
+                                                                double dz, R factor, unsigned long long *__restrict__ cfl_bits,
+                                                                int nplanes) {
   using PL = fft::Plan<M>;
-  constexpr int T = PL::T, E = PL::E, NT = ZB * T;
+  constexpr int TB = xteams<M>();
+  constexpr int T = PL::T, E = PL::E, NT = TB * T;
   constexpr int MO = xmodes<E>();
   // 768-point rows use the four-step transform (fs768_bwd/fwd), 1536-point
   // rows its two-wave form (fs1536_bwd/fwd); the samples between the two
@@ -158,13 +171,26 @@ __global__ __launch_bounds__(ZB *fft::Plan<M>::T, 2) void k_xpass(const cx<R> *_
   constexpr bool FS = M == 768, FS2 = M == 1536;
   constexpr int TEAM_R = FS ? 2 * fft::FS_LDS : (FS2 ? 4 * fft::FS_LDS : fft::lds_reals<M, XXCH>());
   static_assert(!FS2 || (T == 128 && E == 12 && MO == 4), "the two-wave 1536 transform needs 128-lane teams");
-  __shared__ __align__(16) R fl[ZB * TEAM_R];
+  __shared__ __align__(16) R fl[TB * TEAM_R];
   __shared__ double red[NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
-  const int tid = threadIdx.x, zi = tid / T, t = tid % T;
-  const int zb = blockIdx.x % NZB, pl = blockIdx.x / NZB;
+  const int tid = threadIdx.x, t = tid % T;
+  int zb, pl, zi;
+  if constexpr (TB < ZB) {
+    // half z blocks: the halves of tile 8q + x are blocks 16q + x and
+    // 16q + 8 + x, which round-robin dispatch puts on one XCD
+    const int b = blockIdx.x, half = (b >> 3) & 1, tile = ((b >> 4) << 3) | (b & 7);
+    if (tile >= NZB * nplanes) return;  // the whole block: padding of the grid
+    zb = tile % NZB;
+    pl = tile / NZB;
+    zi = TB * half + tid / T;
+  } else {
+    zb = blockIdx.x % NZB;
+    pl = blockIdx.x / NZB;
+    zi = tid / T;
+  }
   const int iy = plane0 + pl - 1;
-  R *fr = fl + zi * TEAM_R;
+  R *fr = fl + (tid / T) * TEAM_R;
   cx<R> *frc = reinterpret_cast<cx<R> *>(fr);
 
   // twiddles in LDS: their reads then wait on the LDS counter, not behind
@@ -570,17 +596,18 @@ template <typename R>
 hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const cx<R> *tw, const Geom &g, int plane0,
                         int nplanes, bool products, int cfl_on, const double *dy, double dx, double dz, R factor,
                         unsigned long long *cfl_bits) {
-  dim3 grid((unsigned)((g.nzd / ZB) * nplanes));
+  const unsigned ntile = (unsigned)((g.nzd / ZB) * nplanes);
   switch (M) {
 #define X(N)                                                                                                          \
   case N: {                                                                                                           \
-    dim3 blk(ZB * fft::Plan<N>::T);                                                                                   \
+    dim3 blk(xteams<N>() * fft::Plan<N>::T);                                                                          \
+    dim3 grid(xteams<N>() < ZB ? 16 * ((ntile + 7) / 8) : ntile);                                                     \
     if (products)                                                                                                     \
       hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,    \
-                         cfl_bits);                                                                                   \
+                         cfl_bits, nplanes);                                                                          \
     else                                                                                                              \
       hipLaunchKernelGGL((k_xpass<N, R, false>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,   \
-                         cfl_bits);                                                                                   \
+                         cfl_bits, nplanes);                                                                          \
     return hipGetLastError();                                                                                         \
   }
     CHAN_FOR_SIZES(X)
