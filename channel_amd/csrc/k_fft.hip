@@ -143,25 +143,27 @@ __device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmo
   }
 }
 
-// teams per x-pass block: one per row of a z block, except for 1536-point
-// rows, whose 128-lane teams hold 29 KB of LDS each in FP64.  Their blocks
-// take half a z block (two rows), so that two blocks share a CU and one
-// block's barriers overlap the other's work (25% less time per launch than
-// four-row blocks at 511x512x511).  The halves read 32 of each 64 contiguous
-// bytes of the z-block interleave; they are numbered 8 blocks apart, so that
-// round-robin dispatch over the 8 XCDs puts both on one XCD and its L2.
-template <int M>
+// teams per x-pass block: one per row of a z block, or one per row of half a
+// z block (TB = 2).  The halves read 32 of each 64 contiguous bytes of the
+// z-block interleave; they are numbered 8 blocks apart, so that round-robin
+// dispatch over the 8 XCDs puts both on one XCD and its L2.  Half blocks
+// serve the 1536-point rows, whose 128-lane teams hold 29 KB of LDS each in
+// FP64: two blocks then share a CU and one block's barriers overlap the
+// other's work (25% less time per launch than four-row blocks at
+// 511x512x511).  FP64 768-point rows gain 2.5% from half blocks; FP32 ones
+// lose 7%, and keep whole z blocks.
+template <int M, typename R>
 constexpr int xteams() {
-  return M == 1536 ? 2 : ZB;
+  return (M == 1536 || (M == 768 && sizeof(R) == 8)) ? 2 : ZB;
 }
 template <int M, typename R, bool PRODUCTS>
-__global__ __launch_bounds__(xteams<M>() * fft::Plan<M>::T, 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
+__global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
                                                                 double dz, R factor, unsigned long long *__restrict__ cfl_bits,
                                                                 int nplanes) {
   using PL = fft::Plan<M>;
-  constexpr int TB = xteams<M>();
+  constexpr int TB = xteams<M, R>();
   constexpr int T = PL::T, E = PL::E, NT = TB * T;
   constexpr int MO = xmodes<E>();
   // 768-point rows use the four-step transform (fs768_bwd/fwd), 1536-point
@@ -179,6 +181,7 @@ __global__ __launch_bounds__(xteams<M>() * fft::Plan<M>::T, 2) void k_xpass(cons
   if constexpr (TB < ZB) {
     // half z blocks: the halves of tile 8q + x are blocks 16q + x and
     // 16q + 8 + x, which round-robin dispatch puts on one XCD
+    static_assert(TB * 2 == ZB, "half z blocks");
     const int b = blockIdx.x, half = (b >> 3) & 1, tile = ((b >> 4) << 3) | (b & 7);
     if (tile >= NZB * nplanes) return;  // the whole block: padding of the grid
     zb = tile % NZB;
@@ -600,8 +603,8 @@ hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const c
   switch (M) {
 #define X(N)                                                                                                          \
   case N: {                                                                                                           \
-    dim3 blk(xteams<N>() * fft::Plan<N>::T);                                                                          \
-    dim3 grid(xteams<N>() < ZB ? 16 * ((ntile + 7) / 8) : ntile);                                                     \
+    dim3 blk(xteams<N, R>() * fft::Plan<N>::T);                                                                       \
+    dim3 grid(xteams<N, R>() < ZB ? 16 * ((ntile + 7) / 8) : ntile);                                                  \
     if (products)                                                                                                     \
       hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,    \
                          cfl_bits, nplanes);                                                                          \
