@@ -103,3 +103,52 @@ def test_config2_10_steps():
     e = rel_l2(g.get_field(), o.get_field())
     print("config2 rel L2 after 10 steps:", e)
     assert e <= 1e-12
+
+
+def test_config3_200_steps_fp64_and_split():
+    """200 steps at 191x384x189 with its production settings (CPI type 1,
+    cflmax 1, so Delta t follows the CFL number): both modes stay finite, the
+    FP64 field keeps continuity, and the split step's Runtimedata columns and
+    profiles stay within 1% of the FP64 step's (the long-run criterion for the
+    split mode, here at the production size)."""
+    from channel_amd.solver import Channel
+    cfg = config3()
+    V0 = seeded_field(cfg)
+    out = {}
+    for prec in (lib.CHAN_FP64, lib.CHAN_SPLIT):
+        print("[case] config3 200 steps, precision", prec, flush=True)
+        g = Channel(cfg, prec)
+        g.set_field(V0)
+        g.initial()
+        g.outstats()
+        rows = g.steps(200)
+        st = g.stats()
+        V = g.get_field() if prec == lib.CHAN_FP64 else None
+        g.close()
+        assert np.all(np.isfinite(rows))
+        out[prec] = (rows, st, V)
+    r64, s64, V = out[lib.CHAN_FP64]
+    r32, s32, _ = out[lib.CHAN_SPLIT]
+    assert np.all(np.isfinite(V))
+    # Delta t adapts: the CFL number (column 9) holds at cflmax
+    np.testing.assert_allclose(r64[:, 9], cfg.cflmax, rtol=1e-12)
+    for col in (0, 1, 2, 3, 4, 5, 6, 10):
+        d = np.max(np.abs(r32[:, col] - r64[:, col])) / np.max(np.abs(r64[:, col]))
+        print("column", col, "split vs fp64 max rel deviation", d)
+        assert d <= 1e-2, (col, d)
+    for key in ("mean", "var", "uv"):
+        d = np.max(np.abs(s32[key] - s64[key])) / np.max(np.abs(s64[key]))
+        print(key, "split vs fp64 max rel deviation", d)
+        assert d <= 1e-2, (key, d)
+    nx, nz, ny = cfg.nx, cfg.nz, cfg.ny
+    v = V[:, :, :, 1].reshape(ny + 3, -1)
+    dv = compact_derivative(cfg, v).reshape(ny + 3, 2 * nz + 1, nx + 1)
+    kx = cfg.alfa0 * np.arange(nx + 1)[None, None, :]
+    kz = cfg.beta0 * np.arange(-nz, nz + 1)[None, :, None]
+    res = 1j * kx * V[:, :, :, 0] + dv + 1j * kz * V[:, :, :, 2]
+    scale = np.abs(kx * V[:, :, :, 0]) + np.abs(dv) + np.abs(kz * V[:, :, :, 2])
+    res[:, nz, 0] = 0.0
+    scale[:, nz, 0] = 1.0
+    rel = np.max(np.abs(res)) / np.max(scale)
+    print("continuity residual / scale after 200 steps:", rel)
+    assert rel <= 1e-12
