@@ -321,14 +321,25 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
     for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
 
-  // CFL maximum over the interior planes (dnsdata.f90:552-556)
+  // CFL maximum over the interior planes (dnsdata.f90:552-556).  FP64 keeps
+  // the reference's divisions; the split mode, whose samples are FP32, takes
+  // the three reciprocals once per row (within an FP64 rounding of them).
   if (cfl_on && iy >= 1 && iy <= g.ny - 1) {
     const double dyv = dy[iy];
     double m = 0.0;
+    if constexpr (sizeof(R) == 4) {
+      const double rx = 1.0 / dx, ry = 1.0 / dyv, rz = 1.0 / dz;
 #pragma unroll
-    for (int s = 0; s < E; s++) {
-      const double c = fabs((double)u[s]) / dx + fabs((double)v[s]) / dyv + fabs((double)w[s]) / dz;
-      m = (c > m || c != c) ? c : m;
+      for (int s = 0; s < E; s++) {
+        const double c = fabs((double)u[s]) * rx + fabs((double)v[s]) * ry + fabs((double)w[s]) * rz;
+        m = (c > m || c != c) ? c : m;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < E; s++) {
+        const double c = fabs((double)u[s]) / dx + fabs((double)v[s]) / dyv + fabs((double)w[s]) / dz;
+        m = (c > m || c != c) ? c : m;
+      }
     }
     block_max_atomic<NT>(m, red, cfl_bits);
   }
