@@ -271,6 +271,15 @@ __global__ __launch_bounds__(256) void k_force(ForceArgs a, const double *__rest
   for (int r = -1; r <= g.ny + 1; r++) o[(size_t)(r + 1) * g.ps] = force_row(a, der, c, a.src[c], r, p, col);
 }
 
+// Blocks of 256 pairs along x of the grid, rounded up to a multiple of 8:
+// blocks (x, y) and (x, y+1) then sit a multiple of 8 apart in dispatch order,
+// so round-robin dispatch puts the neighbouring row chunks of one pair block
+// on one XCD and the halo rows that both read come from its L2 (k_rhs 1.3%
+// faster in FP64 at 191x384x189).  The extra blocks exit at once.
+static unsigned pair_blocks(const Geom &g) {
+  return (unsigned)(((g.npair + 255) / 256 + 7) / 8 * 8);
+}
+
 hipError_t launch_force(hipStream_t st, const ForceArgs &a) {
   dim3 grid((unsigned)((a.g.npair + 255) / 256), 3);
   hipLaunchKernelGGL(k_force, grid, dim3(256), 0, st, a, a.der);
@@ -278,14 +287,14 @@ hipError_t launch_force(hipStream_t st, const ForceArgs &a) {
 }
 
 hipError_t launch_force_terms(hipStream_t st, const ForceArgs &a) {
-  dim3 grid((unsigned)((a.g.npair + 255) / 256), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
+  dim3 grid(pair_blocks(a.g), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
   hipLaunchKernelGGL(k_force_terms, grid, dim3(256), 0, st, a, a.der);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_rhs(hipStream_t st, const RhsArgs<R> &a) {
-  dim3 grid((unsigned)((a.g.npair + 255) / 256), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
+  dim3 grid(pair_blocks(a.g), (unsigned)((a.g.ny - 1 + a.chunk - 1) / a.chunk));
   if (a.force)
     hipLaunchKernelGGL((k_rhs<R, true>), grid, dim3(256), 0, st, a, a.der, a.sc);
   else
