@@ -43,6 +43,18 @@ namespace chan {
 
 constexpr int PF = 4;  // rows of software prefetch in the sequential sweeps
 
+// Block b of a grid of nb = 8m blocks solves column block (b % 8) m + b / 8:
+// round-robin dispatch then gives each XCD a contiguous run of column blocks,
+// so the cache lines that neighbouring blocks share (a block's pair range
+// starts and ends inside a line) stay in one L2.  2% on k_solve_ve at
+// 191x384x189; the extra blocks of the rounded grid exit at once.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+static unsigned xcd_grid(size_t lanes) {
+  return (unsigned)(((lanes + 255) / 256 + 7) / 8 * 8);
+}
+
 // ------------------------------------------------------------ k_solve_ve --
 __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__restrict__ der) {
   // der is a separate restrict argument: the compiler then proves that the
@@ -50,7 +62,7 @@ __global__ __launch_bounds__(256) void k_solve_ve(SolveArgs a, const double *__r
   // the scalar cache, off the vector-memory counter that the sweeps' own
   // prefetches wait on.
   const Geom g = a.g;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   // lane = 4*group + 2*sign + part; group (ix, |iz|) for iz >= 0, part 0 real,
   // 1 imaginary; the group after the last one solves ucor in pair slot npair
   const int grp = gid >> 2, sgn = (gid >> 1) & 1, part = gid & 1;
@@ -465,7 +477,7 @@ __global__ __launch_bounds__(256) void k_flowrate(MeanArgs a) {
 // ------------------------------------------------------------ launchers --
 hipError_t launch_solve(hipStream_t st, const SolveArgs &a) {
   const size_t lanes = (size_t)4 * (a.g.Nx * (a.g.nz + 1) + 1);
-  dim3 grid((unsigned)((lanes + 255) / 256), 2);
+  dim3 grid(xcd_grid(lanes), 2);
   hipLaunchKernelGGL(k_solve_ve, grid, dim3(256), 0, st, a, a.der);
   return hipGetLastError();
 }
