@@ -316,9 +316,13 @@ __global__ __launch_bounds__(256) void k_mean(MeanArgs a) {
 }
 
 // ------------------------------------------------------------------ k_uw --
-__global__ __launch_bounds__(256) void k_uw(UwArgs a, const double *__restrict__ der, const double *__restrict__ d0fac) {
+// Blocks of two waves: the kernel has few waves (one lane per pair and part)
+// and each runs a long sequential sweep, so smaller blocks spread them more
+// evenly over the CUs (5% faster than 256-thread blocks at 191x384x189).
+constexpr int UW_BLOCK = 128;
+__global__ __launch_bounds__(UW_BLOCK) void k_uw(UwArgs a, const double *__restrict__ der, const double *__restrict__ d0fac) {
   const Geom g = a.g;
-  const int gid = blockIdx.x * 256 + threadIdx.x;
+  const int gid = blockIdx.x * UW_BLOCK + threadIdx.x;
   const int p = gid >> 1, part = gid & 1;  // part 0 real, 1 imaginary
   if (p >= g.npair) return;
   const int ix = p / g.Nz, iz = p % g.Nz - g.nz;
@@ -486,7 +490,8 @@ hipError_t launch_mean(hipStream_t st, const MeanArgs &a) {
   return hipGetLastError();
 }
 hipError_t launch_uw(hipStream_t st, const UwArgs &a) {
-  hipLaunchKernelGGL(k_uw, dim3((unsigned)((2 * a.g.npair + 255) / 256)), dim3(256), 0, st, a, a.der, a.d0fac);
+  hipLaunchKernelGGL(k_uw, dim3((unsigned)((2 * a.g.npair + UW_BLOCK - 1) / UW_BLOCK)), dim3(UW_BLOCK), 0, st, a, a.der,
+                     a.d0fac);
   return hipGetLastError();
 }
 hipError_t launch_flowrate(hipStream_t st, const MeanArgs &a) {
