@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
+_LIBS = {}
 
 
 class OrcParams(ctypes.Structure):
@@ -37,13 +37,14 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
-        if not os.path.exists(path):
+def lib(path=None):
+    """The oracle library; `path` names another build of the same source
+    (the mutation tests load deliberately altered copies this way)."""
+    key = path or os.path.join(_HERE, "liboracle.so")
+    if key not in _LIBS:
+        if path is None and not os.path.exists(key):
             build()
-        L = ctypes.CDLL(path)
+        L = ctypes.CDLL(key)
         P = ctypes.c_void_p
         D = ctypes.POINTER(ctypes.c_double)
         I = ctypes.c_int
@@ -81,8 +82,8 @@ def lib():
         L.orc_apply_body_force.argtypes = [P]
         L.orc_body_force_walls.argtypes = [P]
         L.orc_get_body_force.argtypes = [P, D]
-        _LIB = L
-    return _LIB
+        _LIBS[key] = L
+    return _LIBS[key]
 
 
 def _dp(a):
@@ -100,21 +101,22 @@ class Oracle:
 
     SCALARS = ("time", "deltat", "meanpx", "meanpz", "corrpx", "corrpz", "fr1", "fr2", "fr3", "cfl")
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, lib_path=None):
         p = OrcParams()
         for name, _ in OrcParams._fields_:
             v = getattr(cfg, name)
             setattr(p, name, int(v) if name in ("nx", "ny", "nz", "cpi", "cpi_type") else float(v))
         self._p = p
-        self.h = lib().orc_create(ctypes.byref(p))
+        self._L = lib(lib_path)
+        self.h = self._L.orc_create(ctypes.byref(p))
         s = (ctypes.c_int32 * 5)()
-        lib().orc_sizes(self.h, s)
+        self._L.orc_sizes(self.h, s)
         self.nx, self.ny, self.nz, self.nxd, self.nzd = list(s)
         self.shape = (self.ny + 3, 2 * self.nz + 1, self.nx + 1, 3)
 
     def close(self):
         if self.h:
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -123,80 +125,80 @@ class Oracle:
     def set_field(self, V):
         V = np.asfortranarray(V, dtype=np.complex128)
         assert V.shape == self.shape
-        lib().orc_set_field(self.h, _dp(V))
+        self._L.orc_set_field(self.h, _dp(V))
 
     def get_field(self):
         V = np.zeros(self.shape, dtype=np.complex128, order="F")
-        lib().orc_get_field(self.h, _dp(V))
+        self._L.orc_get_field(self.h, _dp(V))
         return V
 
     def scalars(self):
         s = np.zeros(10)
-        lib().orc_get_scalars(self.h, _dp(s))
+        self._L.orc_get_scalars(self.h, _dp(s))
         return dict(zip(self.SCALARS, s))
 
     def set_scalars(self, **kw):
         d = self.scalars()
         d.update(kw)
         s = np.array([d[k] for k in self.SCALARS], dtype=np.float64)
-        lib().orc_set_scalars(self.h, _dp(s))
+        self._L.orc_set_scalars(self.h, _dp(s))
 
     # set-up
     def grid(self):
         y = np.zeros(self.ny + 3)
         dy = np.zeros(self.ny - 1)
-        lib().orc_get_grid(self.h, _dp(y), _dp(dy))
+        self._L.orc_get_grid(self.h, _dp(y), _dp(dy))
         return y, dy
 
     def der(self):
         d = np.zeros((self.ny - 1, 4, 5))
-        lib().orc_get_der(self.h, _dp(d))
+        self._L.orc_get_der(self.h, _dp(d))
         return d  # [iy-1][d0,d1,d2,d4][-2..2]
 
     def bnd(self):
         b = np.zeros((10, 5))
-        lib().orc_get_bnd(self.h, _dp(b))
+        self._L.orc_get_bnd(self.h, _dp(b))
         return dict(zip(("d040", "d140", "d240", "d14m1", "d24m1", "d04n", "d14n", "d24n", "d14np1", "d24np1"), b))
 
     def bc(self):
         b = np.zeros((8, 5))
-        lib().orc_get_bc(self.h, _dp(b))
+        self._L.orc_get_bc(self.h, _dp(b))
         return dict(zip(("v0bc", "v0m1bc", "eta0bc", "eta0m1bc", "vnbc", "vnp1bc", "etanbc", "etanp1bc"), b))
 
     def d0mat(self):
         d = np.zeros((self.ny + 1, 5))
-        lib().orc_get_d0mat(self.h, _dp(d))
+        self._L.orc_get_d0mat(self.h, _dp(d))
         return d
 
     # the step
     def convolutions(self, iy, compute_cfl=False):
         out = np.zeros((6, self.nx + 1, 2 * self.nz + 1), dtype=np.complex128)
-        lib().orc_convolutions(self.h, int(iy), int(compute_cfl), _dp(out))
+        self._L.orc_convolutions(self.h, int(iy), int(compute_cfl), _dp(out))
         return out
 
     def buildrhs(self, ode, compute_cfl=False):
         o = np.ascontiguousarray(ode, dtype=np.float64)
-        lib().orc_buildrhs(self.h, _dp(o), int(compute_cfl))
+        self._L.orc_buildrhs(self.h, _dp(o), int(compute_cfl))
 
     def linsolve(self, lam):
-        lib().orc_linsolve(self.h, float(lam))
+        self._L.orc_linsolve(self.h, float(lam))
 
     def initial(self):
-        lib().orc_initial(self.h)
+        self._L.orc_initial(self.h)
 
     def outstats(self):
         r = np.zeros(11)
-        lib().orc_outstats(self.h, _dp(r))
+        self._L.orc_outstats(self.h, _dp(r))
         return r
 
     def step(self):
         r = np.zeros(11)
-        lib().orc_step(self.h, _dp(r))
+        self._L.orc_step(self.h, _dp(r))
         return r
 
     def steps(self, n):
         r = np.zeros((n, 11))
-        lib().orc_steps(self.h, int(n), _dp(r))
+        self._L.orc_steps(self.h, int(n), _dp(r))
         return r
 
     # body force
@@ -204,45 +206,45 @@ class Oracle:
 
     def set_body_force(self, F):
         if F is None:
-            lib().orc_set_body_force(self.h, None)
+            self._L.orc_set_body_force(self.h, None)
             return
         F = np.asfortranarray(F, dtype=np.complex128)
         assert F.shape == self.shape
-        lib().orc_set_body_force(self.h, _dp(F))
+        self._L.orc_set_body_force(self.h, _dp(F))
 
     def set_body_force_module(self, name, *par):
         """am_f1/am_butterfly(amp, lambdaz_f) or coriolis(omega2, kz_cutoff,
         y_threshold_bot); name None switches the module off."""
         p = np.zeros(3)
         p[:len(par)] = par
-        lib().orc_set_body_force_module(self.h, self.MODULES[name] if name else 0, _dp(p))
+        self._L.orc_set_body_force_module(self.h, self.MODULES[name] if name else 0, _dp(p))
 
     def apply_body_force(self):
-        lib().orc_apply_body_force(self.h)
+        self._L.orc_apply_body_force(self.h)
 
     def body_force_walls(self):
-        lib().orc_body_force_walls(self.h)
+        self._L.orc_body_force_walls(self.h)
 
     def get_body_force(self):
         F = np.zeros(self.shape, dtype=np.complex128, order="F")
-        lib().orc_get_body_force(self.h, _dp(F))
+        self._L.orc_get_body_force(self.h, _dp(F))
         return F
 
     def time_sample(self, nplanes, nix):
         """Seconds per plane of products, per row of rhs, per x-mode column
         of the solve, measured on a bounded sample (modifies the state)."""
         out = np.zeros(3)
-        lib().orc_time_sample(self.h, int(nplanes), int(nix), _dp(out))
+        self._L.orc_time_sample(self.h, int(nplanes), int(nix), _dp(out))
         return out
 
     def yintegr(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
-        return lib().orc_yintegr(self.h, _dp(f))
+        return self._L.orc_yintegr(self.h, _dp(f))
 
     def complexderiv(self, f0):
         f0 = np.ascontiguousarray(f0, dtype=np.complex128)
         f1 = np.zeros_like(f0)
-        lib().orc_complexderiv(self.h, _dp(f0), _dp(f1))
+        self._L.orc_complexderiv(self.h, _dp(f0), _dp(f1))
         return f1
 
 

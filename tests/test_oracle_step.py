@@ -164,3 +164,56 @@ def test_initial_cfl_and_cpi():
     row = o.outstats()
     assert row[10] == pytest.approx(1.0 / (umax / dx), rel=1e-14)  # deltat = cflmax / cfl
     assert row[9] == pytest.approx(1.0, rel=1e-14)
+
+
+# ---------------------------------------------------------------- Couette --
+# Wall velocities u0, uN (dns.in line 8) enter as the (0,0) wall values of the
+# eta system (channel.f90:122-124, linsolve_blocking.inc:65-76); with no
+# pressure gradient the linear profile between them is steady.
+@pytest.mark.parametrize("u0,uN", [(-1.0, 1.0), (0.3, 1.7)])
+def test_laminar_couette_is_a_fixed_point(u0, uN):
+    cfg = tiny(meanflowx=0.0, meanpx=0.0, u0=u0, uN=uN)
+    o = orc.Oracle(cfg)
+    y, _ = o.grid()
+    V0 = np.zeros(cfg.field_shape, dtype=np.complex128, order="F")
+    V0[:, cfg.nz, 0, 0] = u0 + (uN - u0) * y / 2.0
+    o.set_field(V0)
+    rows = o.steps(10)
+    V = o.get_field()
+    assert np.max(np.abs(V - V0)) < 1e-13
+    # Runtimedata: du/dy at the bottom wall and -du/dy at the top wall
+    slope = (uN - u0) / 2.0
+    assert rows[-1, 1] == pytest.approx(slope, rel=1e-12)
+    assert rows[-1, 2] == pytest.approx(-slope, rel=1e-12)
+
+
+def test_couette_walls_hold_with_perturbations():
+    # perturbed Couette: the (0,0) u column takes the wall values exactly and
+    # every other mode keeps no-slip
+    cfg = tiny(meanflowx=0.0, meanpx=0.0, u0=-1.0, uN=1.0)
+    o = orc.Oracle(cfg)
+    y, _ = o.grid()
+    V0 = seeded_field(cfg)
+    V0[:, cfg.nz, 0, 0] = y - 1.0
+    o.set_field(V0)
+    o.steps(5)
+    V = o.get_field()
+    nz, ny = cfg.nz, cfg.ny
+    assert V[1, nz, 0, 0].real == pytest.approx(-1.0, abs=1e-13)
+    assert V[ny + 1, nz, 0, 0].real == pytest.approx(1.0, abs=1e-13)
+    W = V.copy()
+    W[:, nz, 0, 0] = 0.0
+    assert np.max(np.abs(W[[1, ny + 1]])) < 1e-13
+    assert np.max(np.abs(V[:, :, :, 1] - V0[:, :, :, 1])) > 1e-6  # the perturbation evolves
+
+
+def test_couette_plus_poiseuille_is_a_fixed_point():
+    # superposition: y(2-y) kept by dp/dx = 2 nu, plus the wall-driven line
+    cfg = tiny(meanflowx=0.0, meanpx=2.0 / 3000.0, u0=-0.5, uN=0.5)
+    o = orc.Oracle(cfg)
+    y, _ = o.grid()
+    V0 = np.zeros(cfg.field_shape, dtype=np.complex128, order="F")
+    V0[:, cfg.nz, 0, 0] = y * (2.0 - y) + 0.5 * (y - 1.0)
+    o.set_field(V0)
+    o.steps(10)
+    assert np.max(np.abs(o.get_field() - V0)) < 1e-13
