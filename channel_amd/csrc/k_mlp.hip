@@ -292,6 +292,7 @@ __global__ __launch_bounds__(256) void k_mlp_refresh(const float *__restrict__ m
 // ================================================================ C-ABI ==
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/channel_closure.h"
@@ -304,6 +305,7 @@ static_assert(NPARAM == CHAN_MLP_NPARAM, "canonical parameter count");
 
 struct chan_mlp {
   int64_t cap = 0, n = 0;
+  int dev = 0;                // device the model lives on
   hipStream_t st = nullptr;
   __bf16 *X = nullptr;        // [cap][64]
   float *T = nullptr;         // [cap][6]
@@ -321,7 +323,32 @@ struct chan_mlp {
   int losses_cap = 0;
 };
 
+// Staging of chan_mlp_forward_noalloc: the BF16 inputs [CHAN_MLP_NOALLOC_MAX][64]
+// (8 MiB), one copy per GPU for the life of the process.  Host code never
+// passes this name to a kernel: it holds a host-side stand-in, and the device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) __bf16 scratch[(size_t)CHAN_MLP_NOALLOC_MAX * KP];
+
 namespace {
+
+constexpr int MAXDEV = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[MAXDEV] = {};  // device address of `scratch` per device
+
+// Resolves the address of `scratch` on the current device, once per device
+// (called by chan_mlp_create with the model's device current).
+int noalloc_scratch(int dev, void **out) {
+  if (dev < 0 || dev >= MAXDEV) return set_error(CHAN_E_ARG, "device %d beyond the %d the library tracks", dev, MAXDEV);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return set_error((int)(e != hipSuccess ? e : hipErrorInvalidSymbol), "hipGetSymbolAddress of the staging array failed: %s", hipGetErrorString(e));
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
 
 constexpr int MAXBLK = 1024;  // blocks of the per-sample kernels
 constexpr int WBLK = 256;     // blocks of the weight-gradient kernels
@@ -389,9 +416,15 @@ int chan_mlp_create(int64_t capacity, chan_mlp **out) {
   *out = nullptr;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(CHAN_E_NODEV, "no HIP device");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(CHAN_E_NODEV, "no current HIP device");
+  void *scr = nullptr;
+  int rc = noalloc_scratch(dev, &scr);  // once per device, at set-up
+  if (rc) return rc;
+  (void)scr;
   chan_mlp *m = new chan_mlp();
   m->cap = capacity;
-  int rc = 0;
+  m->dev = dev;
   auto bail = [&](int code) {
     chan_mlp_destroy(m);
     return code;
@@ -488,6 +521,30 @@ int chan_mlp_forward(chan_mlp *m, const float *X, int64_t n, float *Y) {
   if (e == hipSuccess) e = hipStreamSynchronize(m->st);
   hipFree(tmp);
   if (e != hipSuccess) return set_error((int)e, "forward: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int chan_mlp_forward_noalloc(chan_mlp *m, const float *dX, int64_t n, float *dY) {
+  if (!m || !dX || !dY || n < 1) return set_error(CHAN_E_ARG, "bad argument");
+  if (n > CHAN_MLP_NOALLOC_MAX)
+    return set_error(CHAN_E_ARG, "chan_mlp_forward_noalloc: %lld rows exceed its staging of %d rows; split the call",
+                     (long long)n, CHAN_MLP_NOALLOC_MAX);
+  int dev = -1;
+  MCHK(hipGetDevice(&dev));
+  if (dev != m->dev) return set_error(CHAN_E_STATE, "chan_mlp_forward_noalloc: device %d is current, the model lives on %d", dev, m->dev);
+  __bf16 *xb = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    xb = static_cast<__bf16 *>(g_scratch[dev]);
+  }
+  if (!xb) return set_error(CHAN_E_STATE, "chan_mlp_forward_noalloc: no staging address for device %d", dev);
+  // the same two kernels and launch shapes as chan_mlp_forward
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((n * KP + 255) / 256)), dim3(256), 0, m->st, dX, xb, n);
+  MCHK(hipGetLastError());
+  hipLaunchKernelGGL((k_mlp_fwd<false>), dim3(nblk_rows(n)), dim3(256), 0, m->st, weights(m), xb,
+                     (const float *)nullptr, (int)n, 0.0f, dY, (__bf16 *)nullptr, (__bf16 *)nullptr, (float *)nullptr);
+  MCHK(hipGetLastError());
+  MCHK(hipStreamSynchronize(m->st));
   return 0;
 }
 

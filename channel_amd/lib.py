@@ -20,6 +20,8 @@ CHAN_E_NOTFINITE = -3
 CHAN_E_CPITYPE = -4
 CHAN_E_STATE = -5
 CHAN_E_NODEV = -6
+# rows that chan_mlp_forward_noalloc stages (include/channel_closure.h)
+CHAN_MLP_NOALLOC_MAX = 65536
 
 _lib = None
 
@@ -95,6 +97,7 @@ SIGNATURES = {
     "chan_mlp_get_params": (ctypes.c_int, [_P, _F]),
     "chan_mlp_set_data": (ctypes.c_int, [_P, _F, _F, ctypes.c_int64]),
     "chan_mlp_forward": (ctypes.c_int, [_P, _F, ctypes.c_int64, _F]),
+    "chan_mlp_forward_noalloc": (ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "chan_mlp_grad": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _F, _D]),
     "chan_mlp_train": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _F, _D]),
     "chan_mlp_bench_forward": (ctypes.c_int, [_P, _I32, _D]),

@@ -47,6 +47,21 @@ int chan_mlp_set_data(chan_mlp *m, const float *X, const float *T, int64_t n);
 /* Inference on host rows X[n][46] -> Y[n][6]; n <= capacity. */
 int chan_mlp_forward(chan_mlp *m, const float *X, int64_t n, float *Y);
 
+/* Inference that allocates nothing: device rows dX[n][46] (FP32) ->
+ * dY[n][6] (FP32), both device pointers of the model's GPU, which must be
+ * current.  Same kernels and results as chan_mlp_forward; the BF16 inputs
+ * are staged in a static device array of CHAN_MLP_NOALLOC_MAX rows (8 MiB,
+ * held by every GPU the library runs on until the process ends) instead of
+ * device memory allocated per call.  n > CHAN_MLP_NOALLOC_MAX is rejected
+ * with CHAN_E_ARG: split the call.  Runs on the model's stream and returns
+ * when it is done.  All calls on one GPU share that array: calls that
+ * overlap on one GPU (two host threads, two models) race.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). */
+#define CHAN_MLP_NOALLOC_MAX 65536
+int chan_mlp_forward_noalloc(chan_mlp *m, const float *dX, int64_t n, float *dY);
+
 /* Loss sum((y - t)^2) / (6 count) over resident samples [first, first+count)
  * and its gradient in the canonical layout (FP32 accumulation of BF16
  * operands; the stored activations and dz are BF16). */
