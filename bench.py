@@ -68,35 +68,49 @@ def algorithmic_bytes(cfg, nxd, nzd, c=16):
     return per, step
 
 
-def cpu_baseline(cfg, V, nplanes=None, nix=None):
-    """The oracle (plain C restatement of the reference step, OpenMP over
-    planes and x modes) timed on the host cores on a bounded sample of the
-    same workload, extrapolated to RK3 steps/s.  By default the sample is
-    nearly one whole substep (products of the ny-1 interior planes, the rhs
-    of rows 3..ny-3, the solves of the x-mode columns 1..nx): about 10 s on
-    16 cores at 191x384x189."""
-    if nplanes is None:
-        nplanes = cfg.ny - 1
-    if nix is None:
-        nix = cfg.nx
+def cpu_threads():
+    """Threads for the oracle: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS when that is set (the GPU box sets it to the host share
+    that goes with one GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return min(avail, int(env)), avail
+    return avail, avail
+
+
+def cpu_baseline(cfg, V):
+    """The oracle (plain-C restatement of the reference step: OpenMP threads
+    over y planes for the products and the rhs and over (kx, kz) pairs for
+    the solves, its own mixed-radix FFT in place of FFTW) timed on the host
+    cores for one whole RK3 step of the same workload: the initial CFL pass
+    sets Delta t, then one step is timed by the wall clock (about 30 s on 16
+    cores at 191x384x189)."""
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    threads, avail = cpu_threads()
     orc.set_threads(threads)
     o = orc.Oracle(cfg)
     o.set_field(V)
-    t0 = time.time()
-    tc, tr, ts = o.time_sample(nplanes, nix)
-    wall = time.time() - t0
+    o.initial()
+    o.outstats()
+    t0 = time.perf_counter()
+    row = o.step()
+    wall = time.perf_counter() - t0
     o.close()
-    Ny, Nyi, Nx = cfg.ny + 3, cfg.ny - 1, cfg.nx + 1
-    t_step = 3 * (Ny * tc + Nyi * tr + Nx * ts)
+    if not np.all(np.isfinite(row)):
+        raise SystemExit("non-finite Runtimedata in the CPU baseline step")
     return {
-        "value": 1.0 / t_step,
+        "value": 1.0 / wall,
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": "products of %d planes, rhs of %d rows, solve of %d x-mode columns of one substep (%.1f s wall), "
-                  "scaled to %d planes, %d rows, %d columns x 3 substeps" % (nplanes, nplanes - 4, nix, wall, Ny, Nyi, Nx),
+        "sample": "one whole RK3 step (3 substeps, %.1f s wall) of the oracle: the plain-C restatement of the "
+                  "reference step with its own mixed-radix FFT, not FFTW; %d OpenMP threads over y planes (products, "
+                  "rhs) and (kx,kz) pairs (solves) on %d usable host CPUs (nproc %d)"
+                  % (wall, threads, avail, os.cpu_count() or 0),
     }
 
 
