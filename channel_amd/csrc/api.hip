@@ -13,6 +13,7 @@
 
 #include "../../include/channel_hip.h"
 #include "chan_internal.hpp"
+#include "ctx_internal.hpp"
 
 using namespace chan;
 
@@ -141,46 +142,6 @@ hipError_t launch_to_host_layout(hipStream_t st, const cx<double> *in, cx<double
   return hipGetLastError();
 }
 }  // namespace chan
-
-struct chan_ctx {
-  chan_params prm;
-  HostSetup hs;
-  Geom g;
-  hipStream_t st = nullptr;
-  // device state
-  cx<double> *V = nullptr, *rhs = nullptr, *col00 = nullptr;
-  void *old = nullptr;  // explicit terms: complex FP64, or FP32 in the split mode
-  double2 *L = nullptr;
-  // transform intermediates and twiddles: complex FP64, or complex FP32 in
-  // the split mode
-  bool split = false;
-  void *A = nullptr, *B = nullptr, *P = nullptr, *twz = nullptr, *twx = nullptr;
-  double *der = nullptr, *d0fac = nullptr, *y = nullptr, *dy = nullptr;
-  double *stat = nullptr;  // statistics: [3][ny+1] mean in, [7][ny+1] profiles out
-  // body force: static field [3][Ny][ps] and feedback gains [3][Ny][Nz+1]
-  cx<double> *F0 = nullptr;
-  double *gain = nullptr;
-  int src[3] = {-1, -1, -1};
-  // the force of the last buildrhs, kept on request: [3][Ny][ps]; kept_state
-  // 0 = nothing kept yet, 1 = Fkeep holds it, 2 = that buildrhs had no force
-  cx<double> *Fkeep = nullptr;
-  int kept_state = 0;
-  DevScalars *sc = nullptr;
-  // host scalars
-  double time = 0, deltat = 0;
-  bool field_set = false;
-  // timing
-  bool timing = false;
-  struct Rec {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<Rec> pend;
-  std::vector<hipEvent_t> pool;
-  std::vector<std::string> names;
-  std::vector<long long> counts;
-  std::vector<double> ms;
-};
 
 namespace {
 
@@ -384,6 +345,7 @@ int chan_create(const chan_params *p, chan_ctx **out) {
 int chan_destroy(chan_ctx *c) {
   if (!c) return 0;
   if (c->st) hipStreamSynchronize(c->st);
+  if (c->les) les_free(c->les);
   for (auto &r : c->pend) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);

@@ -409,6 +409,13 @@ int grad_pass(chan_mlp *m, int64_t first, int count) {
 
 }  // namespace
 
+hipError_t chan::mlp::forward_device(chan_mlp *m, const __bf16 *X, int64_t n, float *Y, hipStream_t st) {
+  if (n < 1 || n > ((int64_t)1 << 31) / KP) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_mlp_fwd<false>), dim3(nblk_rows(n)), dim3(256), 0, st, weights(m), X, (const float *)nullptr,
+                     (int)n, 0.0f, Y, (__bf16 *)nullptr, (__bf16 *)nullptr, (float *)nullptr);
+  return hipGetLastError();
+}
+
 extern "C" {
 
 int chan_mlp_create(int64_t capacity, chan_mlp **out) {

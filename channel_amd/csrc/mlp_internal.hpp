@@ -61,3 +61,12 @@ inline int canon_to_padded(int c) {
 
 }  // namespace mlp
 }  // namespace chan
+
+struct chan_mlp;
+namespace chan {
+namespace mlp {
+// inference of model m on n resident BF16 rows X[n][64] -> Y[n][6] FP32,
+// enqueued on stream st (k_mlp.hip)
+hipError_t forward_device(chan_mlp *m, const __bf16 *X, int64_t n, float *Y, hipStream_t st);
+}  // namespace mlp
+}  // namespace chan

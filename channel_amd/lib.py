@@ -101,6 +101,9 @@ SIGNATURES = {
     "chan_mlp_grad": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, _F, _D]),
     "chan_mlp_train": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_float, _F, _D]),
     "chan_mlp_bench_forward": (ctypes.c_int, [_P, _I32, _D]),
+    "chan_les_samples": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "chan_les_features": (ctypes.c_int, [_P, _F]),
+    "chan_les_closure_force": (ctypes.c_int, [_P, _P, _F]),
 }
 
 

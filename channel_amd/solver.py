@@ -154,6 +154,27 @@ class Channel:
         lib.check(self._L.chan_get_applied_body_force(self.h, lib.dptr(F)))
         return F
 
+    # ------------------------------------------------ learned closure --
+    def les_samples(self):
+        n = ctypes.c_int64()
+        lib.check(self._L.chan_les_samples(self.h, ctypes.byref(n)))
+        return n.value
+
+    def les_features(self):
+        """Closure features (n, 46) float32 of the resident field
+        (include/channel_closure.h)."""
+        X = np.zeros((self.les_samples(), 46), dtype=np.float32)
+        lib.check(self._L.chan_les_features(self.h, lib.fptr(X)))
+        return X
+
+    def les_closure_force(self, mlp, norm):
+        """Static body force := -div(tau) of the MLP's stresses on the
+        resident field, on the device; norm = [46 xmean, 46 xscale, 6 tmean,
+        6 tscale]."""
+        nv = np.ascontiguousarray(norm, dtype=np.float32)
+        assert nv.size == 104
+        lib.check(self._L.chan_les_closure_force(self.h, mlp.h, lib.fptr(nv)))
+
     def steps(self, n):
         r = np.zeros((n, 11))
         if n:

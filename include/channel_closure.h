@@ -23,6 +23,8 @@
 
 #include <stdint.h>
 
+#include "channel_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -76,6 +78,27 @@ int chan_mlp_train(chan_mlp *m, int32_t steps, int32_t batch, float lr, float *l
 /* Timed inference over the resident samples: `reps` forward passes, average
  * milliseconds per pass in *ms. */
 int chan_mlp_bench_forward(chan_mlp *m, int32_t reps, double *ms);
+
+/* ---- the closure inside a coarse (LES) channel step, on the device ----
+ * Samples are the points (iy, x, z) of rows iy = 0..ny of the context's
+ * dealiased physical grid 2nxd x nzd, in that order: n = (ny+1) 2nxd nzd
+ * (chan_les_samples).  Features, 46 per sample: the nine velocity gradients
+ * du_c/dx_d (index 3c + d; d = x, y, z; x and z derivatives spectral, y
+ * derivatives second-order differences on the stretched rows) at rows iy,
+ * iy-2, iy-1, iy+1, iy+2 (clamped to 0..ny), then y+ = min(y-ymin, ymax-y)
+ * u_tau / nu with u_tau from the mean wall shear of the (0,0) mode at both
+ * walls.  Stresses (the MLP's outputs): tau_xx, tau_yy, tau_zz, tau_xy,
+ * tau_xz, tau_yz. */
+int chan_les_samples(const chan_ctx *c, int64_t *n);
+/* Features X[n][46] (FP32, host) of the context's resident field. */
+int chan_les_features(chan_ctx *c, float *X);
+/* Sets the context's static body force (the F0 of chan_set_body_force) to
+ * -div(tau), tau = MLP(m) of the normalised features of the resident field:
+ * inputs (X - xmean) / xscale, stresses tau = Y * tscale + tmean, with
+ * norm = [46 xmean][46 xscale][6 tmean][6 tscale].  Everything stays on the
+ * device; the next chan_step / chan_buildrhs applies the force with the
+ * wall treatment of chan_set_body_force. */
+int chan_les_closure_force(chan_ctx *c, chan_mlp *m, const float *norm);
 
 #ifdef __cplusplus
 }
