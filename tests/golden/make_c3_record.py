@@ -1,0 +1,64 @@
+"""Writes c3_record.npz: a fingerprint of the oracle's 191x384x189 FP64 run
+from the benchmark's seeded field (seed 20261202): the initial CFL pass
+(which sets Delta t = cflmax/CFL), then 10 RK3 steps with CPI type 1.
+
+Contents: 'rows' -- the Runtimedata row of the initial pass and of steps
+1..10 (11 x 11); 'idx' -- (n, 4) entries (row, iz + nz, ix, component) of
+the field in the reference layout: every row of the modes |kx|, |kz| <= 2
+(index ix <= 2, |iz| <= 2) and 4096 entries drawn uniformly (numpy seed 5);
+'v1', 'v10' -- those entries after steps 1 and 10 (complex128).
+
+The oracle needs about 27 s per step on 16 cores; run from the repository
+root: python tests/golden/make_c3_record.py [threads] [out.npz]
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from cases import config3, seeded_field  # noqa: E402
+
+
+def sample_index(cfg):
+    nz = cfg.nz
+    Ny, Nz, Nx = cfg.ny + 3, 2 * cfg.nz + 1, cfg.nx + 1
+    rows = np.arange(Ny)
+    low = [(r, nz + iz, ix, c) for r in rows for ix in range(3) for iz in range(-2, 3) for c in range(3)]
+    rng = np.random.default_rng(5)
+    rnd = np.stack([rng.integers(0, Ny, 4096), rng.integers(0, Nz, 4096), rng.integers(0, Nx, 4096),
+                    rng.integers(0, 3, 4096)], axis=1)
+    return np.concatenate([np.array(low), rnd]).astype(np.int64)
+
+
+def take(V, idx):
+    return V[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]]
+
+
+def main():
+    from oracle import oracle as orc
+    threads = int(sys.argv[1]) if len(sys.argv) > 1 else (os.cpu_count() or 1)
+    out = sys.argv[2] if len(sys.argv) > 2 else os.path.join(os.path.dirname(os.path.abspath(__file__)), "c3_record.npz")
+    orc.set_threads(threads)
+    cfg = config3()
+    idx = sample_index(cfg)
+    o = orc.Oracle(cfg)
+    o.set_field(seeded_field(cfg))
+    o.initial()
+    rows = [o.outstats()]
+    res = {}
+    t0 = time.time()
+    for s in range(1, 11):
+        rows.append(o.step())
+        if s in (1, 10):
+            res["v%d" % s] = take(o.get_field(), idx)
+        print("step", s, "%.0f s" % (time.time() - t0), flush=True)
+    np.savez_compressed(out, rows=np.array(rows), idx=idx.astype(np.int16), **res)
+
+
+if __name__ == "__main__":
+    main()

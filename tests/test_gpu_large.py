@@ -186,3 +186,94 @@ def test_config3_one_step_vs_oracle():
         rtol = 1e-11 if prec == lib.CHAN_FP64 else 1e-5
         np.testing.assert_array_less(np.abs(rg0 - ro0), rtol * np.maximum(np.abs(ro0), 1.0))
         np.testing.assert_array_less(np.abs(rg - ro), rtol * scale)
+
+
+def test_config3_ten_steps_vs_oracle_record():
+    """10 RK3 steps at 191x384x189 from the benchmark's field (initial CFL
+    pass, adaptive Delta t, CPI type 1) against the oracle's record
+    (tests/golden/c3_record.npz, written by make_c3_record.py): the FP64
+    step's relative L2 on the recorded entries of each velocity component
+    <= 1e-12 after steps 1 and 10 and its Runtimedata rows within 1e-11; the
+    split step's errors are reported per component and per Runtimedata
+    column, and held to 1e-5 and 1e-6."""
+    from channel_amd.solver import Channel
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_record.npz")
+    rec = np.load(path)
+    idx = rec["idx"].astype(np.int64)
+    cfg = config3()
+    V0 = seeded_field(cfg)
+    print("[case] config3 10 steps vs oracle record", flush=True)
+    for prec, tol, rtol in ((lib.CHAN_FP64, 1e-12, 1e-11), (lib.CHAN_SPLIT, 1e-5, 1e-6)):
+        g = Channel(cfg, prec)
+        g.set_field(V0)
+        g.initial()
+        rows = [g.outstats()]
+        for s in range(1, 11):
+            rows.append(g.step())
+            if s in (1, 10):
+                V = g.get_field()
+                got = V[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]]
+                want = rec["v%d" % s]
+                for c in range(3):
+                    sel = idx[:, 3] == c
+                    e = np.linalg.norm(got[sel] - want[sel]) / np.linalg.norm(want[sel])
+                    print("precision", prec, "step", s, "component", "uvw"[c], "rel L2 %.3e" % e, flush=True)
+                    assert e <= tol, (prec, s, c, e)
+        g.close()
+        rows = np.array(rows)
+        dev = np.abs(rows - rec["rows"]) / np.maximum(np.abs(rec["rows"]), 1.0)
+        print("precision", prec, "Runtimedata deviation per column", np.max(dev, axis=0), flush=True)
+        assert np.all(dev <= rtol), (prec, np.max(dev, axis=0))
+
+
+def config4():
+    from channel_amd.dns import DnsConfig
+    return DnsConfig(nx=511, ny=512, nz=511, alfa0=0.5, beta0=1.0, ni=1.0 / 20000.0, a=1.5,
+                     meanflowx=4.0 / 3.0, deltat=0.002)
+
+
+def test_config4_fp64_and_split():
+    """511x512x511 (about 60 GB resident per step context): three FP64 and
+    three split steps from the seeded field.  Both stay finite, the FP64
+    field keeps kx = 0 Hermitian, v = 0 in the (0,0) mode and continuity on
+    2000 sampled pairs, and the split step's Runtimedata columns stay within
+    1% of the FP64 step's."""
+    from channel_amd.solver import Channel
+    cfg = config4()
+    print("[case] config4 3 steps fp64 and split", flush=True)
+    V0 = seeded_field(cfg)
+    rows = {}
+    for prec in (lib.CHAN_FP64, lib.CHAN_SPLIT):
+        g = Channel(cfg, prec)
+        g.set_field(V0)
+        rows[prec] = np.array([g.step() for _ in range(3)])
+        if prec == lib.CHAN_FP64:
+            V = g.get_field()
+        g.close()
+        assert np.all(np.isfinite(rows[prec])), prec
+        print("precision", prec, "last row", rows[prec][-1], flush=True)
+    del V0
+    assert np.all(np.isfinite(V))
+    nx, nz, ny = cfg.nx, cfg.nz, cfg.ny
+    P0 = V[:, :, 0, :]
+    assert np.max(np.abs(P0[:, ::-1, :] - np.conj(P0))) <= 1e-13 * np.max(np.abs(P0))
+    assert np.max(np.abs(V[:, nz, 0, 1])) == 0.0
+    rng = np.random.default_rng(9)
+    ixs, izs = rng.integers(0, nx + 1, 2000), rng.integers(-nz, nz + 1, 2000)
+    keep = ~((ixs == 0) & (izs == 0))
+    ixs, izs = ixs[keep], izs[keep]
+    v = V[:, nz + izs, ixs, 1]
+    dv = compact_derivative(cfg, v)
+    kx = cfg.alfa0 * ixs[None, :]
+    kz = cfg.beta0 * izs[None, :]
+    u, w = V[:, nz + izs, ixs, 0], V[:, nz + izs, ixs, 2]
+    res = 1j * kx * u + dv + 1j * kz * w
+    scale = np.abs(kx * u) + np.abs(dv) + np.abs(kz * w)
+    rel = np.max(np.abs(res)) / np.max(scale)
+    print("config4 continuity residual / scale:", rel, flush=True)
+    assert rel <= 1e-12
+    dev = np.abs(rows[lib.CHAN_SPLIT] - rows[lib.CHAN_FP64]) / np.maximum(np.abs(rows[lib.CHAN_FP64]), 1e-300)
+    # columns: time, wall shears, flow rates, dp/dx, CFL, dt (dp/dz and the z flow rate stay ~0)
+    cols = [1, 2, 5, 6, 9, 10]
+    print("config4 split vs FP64 Runtimedata deviation", np.max(dev[:, cols], axis=0), flush=True)
+    assert np.all(dev[:, cols] <= 1e-2)
