@@ -260,7 +260,14 @@ int chan_create(const chan_params *p, chan_ctx **out) {
   // the x pass holds at most (E+2)/3 x modes per thread, i.e. Nx <= 2nxd/3,
   // which fftFIT's nxd >= 3(nx+1)/2 guarantees
   if (!fft_size_supported(h.nzd) || !fft_size_supported(2 * h.nxd) || h.nzd % ZB != 0 || 3 * (p->nx + 1) > 2 * h.nxd) {
-    int rc = fail(CHAN_E_SIZE, "transform sizes nzd=%d, 2nxd=%d are not supported", h.nzd, 2 * h.nxd);
+    int rc;
+    if (3 * (p->nx + 1) > 2 * h.nxd)
+      rc = fail(CHAN_E_SIZE,
+                "nx=%d: the x pass needs 3(nx+1) <= 2nxd, but fftFIT keeps nxd=%d (3(nx+1)/2 rounded down is "
+                "already 2^k or 3*2^k), so 3(nx+1)=%d > 2nxd=%d; choose another nx (nx=%d or %d)",
+                p->nx, h.nxd, 3 * (p->nx + 1), 2 * h.nxd, p->nx - 1, p->nx + 1);
+    else
+      rc = fail(CHAN_E_SIZE, "transform sizes nzd=%d, 2nxd=%d are not supported", h.nzd, 2 * h.nxd);
     setup_free(c->hs);
     delete c;
     return rc;

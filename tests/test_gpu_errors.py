@@ -50,3 +50,13 @@ def test_unsupported_transform_size():
     rc = lib.load().chan_create(ctypes.byref(p), ctypes.byref(h))
     assert rc == lib.CHAN_E_SIZE and not h.value
     assert b"not supported" in lib.load().chan_last_error()
+
+
+def test_nx_beyond_the_x_pass():
+    # nx = 2: fftFIT keeps nxd = 4, and 3(nx+1) = 9 > 2nxd = 8
+    p = lib.params_from(tiny(nx=2, ny=8, nz=3))
+    h = ctypes.c_void_p()
+    rc = lib.load().chan_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == lib.CHAN_E_SIZE and not h.value
+    msg = lib.load().chan_last_error()
+    assert b"3(nx+1) <= 2nxd" in msg and b"nx=1 or 3" in msg

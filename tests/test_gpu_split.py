@@ -112,3 +112,56 @@ def test_split_four_step_x_transform_vs_oracle(nx):
     e = rel_l2(g.get_field(), o.get_field())
     print("split rel L2 after 3 steps:", e)
     assert 1e-14 < e <= SPLIT_TOL
+
+
+def _field_stats(V, nz):
+    """Per-component fields and the statistics of rows 0..ny (plane means,
+    rms about them, uv; weight 2 off kx = 0, postpro/history/var_history.f90)."""
+    P = V[1:-1]
+    m = P[:, nz, 0, :].real
+    f = np.full(P.shape[2], 2.0)
+    f[0] = 1.0
+    Q = P.copy()
+    Q[:, nz, 0, :] -= m
+    rms = np.sqrt(np.einsum("x,yzxc->cy", f, np.abs(Q) ** 2))
+    uv = np.einsum("x,yzx->y", f, (Q[..., 0] * np.conj(Q[..., 1])).real)
+    return {"mean_u": m[:, 0], "rms_u": rms[0], "rms_v": rms[1], "rms_w": rms[2], "uv": uv}
+
+
+def test_split_errors_per_field_and_statistic():
+    """The split step's error for every field and statistic: u, v, w
+    (relative L2 per component), the mean-velocity, rms and uv profiles
+    (largest deviation over the profile's largest value) and the 11
+    Runtimedata columns, against the FP64 oracle after 1 and 10 steps and
+    against the FP64 HIP step after 100 (31x64x31)."""
+    from channel_amd.solver import Channel
+    cfg = config1()
+    say("split per field and statistic")
+    V = seeded_field(cfg)
+    g = split_channel(cfg, V)
+    o = orc.Oracle(cfg)
+    o.set_field(V)
+    d = Channel(cfg)
+    d.set_field(V)
+    report = []
+    done = 0
+    for n, tol in ((1, SPLIT_TOL), (10, SPLIT_TOL), (100, 1e-5)):
+        rg = g.steps(n - done)[-1]
+        rref = (o.steps(n - done) if n <= 10 else d.steps(n - done))[-1]
+        if n <= 10:
+            d.steps(n - done)
+        done = n
+        Vg = g.get_field()
+        Vr = o.get_field() if n <= 10 else d.get_field()
+        errs = {"uvw"[c]: rel_l2(Vg[..., c], Vr[..., c]) for c in range(3)}
+        sg, sr = _field_stats(Vg, cfg.nz), _field_stats(Vr, cfg.nz)
+        for k in sg:
+            errs[k] = float(np.max(np.abs(sg[k] - sr[k])) / np.max(np.abs(sr[k])))
+        cols = np.abs(rg - rref) / np.maximum(np.abs(rref), 1.0)
+        report.append((n, errs, cols))
+        print("after %d steps vs %s:" % (n, "oracle" if n <= 10 else "FP64 HIP step"),
+              " ".join("%s %.2e" % kv for kv in errs.items()), "| Runtimedata", np.array2string(cols, precision=1))
+        for k, e in errs.items():
+            assert e <= tol, (n, k, e)
+        assert np.all(cols <= tol), (n, cols)
+    o.close()
