@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libchannel_hip.so")
 ARCH = os.environ.get("CHAN_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["setup.cpp", "k_fft.hip", "k_rhs.hip", "k_solve.hip", "k_stats.hip", "k_mlp.hip", "k_les.hip", "api.hip"]
-HEADERS = ["fft.hpp", "chan_internal.hpp", "ctx_internal.hpp", "mlp_internal.hpp"]
+HEADERS = ["fft.hpp", "setup.hpp", "chan_internal.hpp", "ctx_internal.hpp", "mlp_internal.hpp"]
 
 
 def hipcc():

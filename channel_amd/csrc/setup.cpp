@@ -11,7 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "chan_internal.hpp"
+#include "setup.hpp"
 
 namespace chan {
 
