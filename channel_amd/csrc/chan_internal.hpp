@@ -27,16 +27,6 @@ struct Geom {
   double alfa0, beta0;
 };
 
-// Device copies of the set-up.
-struct DevSetup {
-  const double *der;    // [(ny-1)][4][5]
-  const double *d0fac;  // [(ny+1)][5]
-  const double *y;      // [ny+3]
-  const double *dy;     // [ny+1]
-  double bnd[10][5];
-  double bc[8][5];
-};
-
 // Scalars that live on the device between kernels (the (0,0) mode writes
 // them in linsolve, buildrhs reads meanpx/meanpz).
 struct DevScalars {

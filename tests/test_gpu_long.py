@@ -1,7 +1,10 @@
 """Long runs at 31x64x31 against the oracle's 2000-step record
 (tests/golden/long_run_c1.npz, written by tests/golden/make_long_run.py):
 Runtimedata columns over the run and the statistics profiles (plane means,
-variances, uv) after 200 and 2000 steps.
+variances, uv) after 200 and 2000 steps.  At 95x128x95 the oracle's record
+(long_run_c2.npz) holds the profiles of the steps that its run reached
+('last_step': the oracle takes about 13 s per step there on 6 cores), and
+the GPU run is compared at each of them.
 
 Criterion: deviations relative to each profile's (or column's) largest
 value.  The 1% criterion for long runs is met with a wide margin, so the
@@ -14,12 +17,13 @@ import os
 import numpy as np
 import pytest
 
-from cases import config1, seeded_field
+from cases import config1, config2, seeded_field
 from channel_amd import lib
 
 pytestmark = pytest.mark.gpu
 
 RECORD = os.path.join(os.path.dirname(__file__), "golden", "long_run_c1.npz")
+RECORD_C2 = os.path.join(os.path.dirname(__file__), "golden", "long_run_c2.npz")
 
 
 def say(*a):
@@ -62,6 +66,34 @@ def test_2000_steps_vs_oracle_record(precision, bound):
         print("column", col, "max rel deviation", d)
         assert d <= bound, (col, d)
     for n in (200, 2000):
+        d = profile_dev(st[n], ref, n)
+        print("profiles after", n, "steps:", d)
+        assert max(d.values()) <= bound, (n, d)
+
+
+@pytest.mark.parametrize("precision,bound", [(lib.CHAN_FP64, 1e-8), (lib.CHAN_SPLIT, 1e-4)], ids=["fp64", "split"])
+def test_c2_long_run_vs_oracle_record(precision, bound):
+    from channel_amd.solver import Channel
+    ref = np.load(RECORD_C2, allow_pickle=False)
+    last = int(ref["last_step"])
+    marks = sorted(n for n in (200, 1000, 2000) if "mean_%d" % n in ref.files)
+    say("%d steps at 95x128x95, precision" % last, precision)
+    cfg = config2()
+    g = Channel(cfg, precision)
+    g.set_field(seeded_field(cfg))
+    rows, st, done = [], {}, 0
+    for n in marks:
+        rows.append(g.steps(n - done))
+        done = n
+        st[n] = g.stats()
+    rows = np.vstack(rows)
+    sel = ref["row_steps"] - 1
+    r, rr = rows[sel], ref["rows"]
+    for col in (0, 1, 2, 3, 4, 5, 6, 9, 10):
+        d = dev(r[:, col], rr[:, col])
+        print("column", col, "max rel deviation", d)
+        assert d <= bound, (col, d)
+    for n in marks:
         d = profile_dev(st[n], ref, n)
         print("profiles after", n, "steps:", d)
         assert max(d.values()) <= bound, (n, d)
