@@ -3,7 +3,6 @@ chan_get_body_force) against the oracle, which restates the reference's
 force terms in buildrhs (dnsdata.f90:616-659) and its three modules
 (body_forces/*) literally.  FP64 bar as for the plain step: relative L2 field
 error <= 1e-12 after 1 and 10 steps; the split mode to 1e-6."""
-import os
 
 import numpy as np
 import pytest
@@ -16,7 +15,6 @@ from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-orc.set_threads(min(16, os.cpu_count() or 1))
 TOL = 1e-12
 
 

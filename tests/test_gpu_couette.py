@@ -1,7 +1,6 @@
 """HIP path: Couette flow through the wall velocities u0, uN (dns.in line 8,
 channel.f90:122-124), the explicit nonlinear terms of buildrhs against the
 Kim-Moin-Moser form (dnsdata.f90:633-661), and bitwise reruns of the step."""
-import os
 
 import numpy as np
 import pytest
@@ -14,7 +13,6 @@ from test_oracle_nonlinear import _case, _expected, _field
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
-orc.set_threads(min(16, os.cpu_count() or 1))
 
 
 def say(*a):

@@ -18,7 +18,6 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "channel_amd", "host", "channel")
-orc.set_threads(min(16, os.cpu_count() or 1))
 
 
 def run_host(cwd):

@@ -20,7 +20,6 @@ from channel_amd import lib
 from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
-orc.set_threads(min(16, os.cpu_count() or 1))
 
 
 def compact_derivative(cfg, F):

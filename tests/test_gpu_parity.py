@@ -2,7 +2,6 @@
 seeded inputs.  FP64 bar: relative L2 field error <= 1e-12 after 1 and 10
 RK3 steps; intermediate quantities (rhs, scalars, Runtimedata columns) are
 held to the same order.  FFTs against numpy at <= 1e-15 relative L2."""
-import os
 
 import numpy as np
 import pytest
@@ -18,7 +17,6 @@ RK2 = (120.0 / 8.0, 50.0 / 8.0, 34.0 / 8.0)
 RK3 = (120.0 / 20.0, 90.0 / 20.0, 50.0 / 20.0)
 TOL = 1e-12
 
-orc.set_threads(min(16, os.cpu_count() or 1))
 
 
 def say(*a):

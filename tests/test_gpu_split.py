@@ -4,7 +4,6 @@ products read by K4); FP64 state, stencil arithmetic, solves, (0,0) mode,
 flow rates and reductions.  Tolerances are set by FP32 rounding of the
 nonlinear term: relative L2 field error <= 1e-6 after 1 and 10 steps and
 <= 1e-5 after 100, Runtimedata columns to 1e-6 of max(|value|, 1)."""
-import os
 
 import numpy as np
 import pytest
@@ -15,7 +14,6 @@ from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
-orc.set_threads(min(16, os.cpu_count() or 1))
 SPLIT_TOL = 1e-6
 
 
