@@ -138,6 +138,56 @@ __device__ __forceinline__ void team_sync() {
   }
 }
 
+// 12 = 4 x 3: n = 3 n1 + n2, q = q1 + 4 q2;
+// y_q = sum_n2 w3^(n2 q2) w12^(n2 q1) sum_n1 w4^(n1 q1) a_(3 n1 + n2).
+// MID0: inputs 3..8 are known to be zero (the backward transforms of the x
+// pass at nx + 1 <= 192 hold no modes there), so each DFT-4 has inputs
+// (x, 0, 0, y) only; the compiler cannot drop additions of zero itself,
+// which would change the sign of a zero result.
+template <int RAD, int S, typename R>
+__device__ __forceinline__ void dft(cx<R> *a);
+
+template <int S, bool MID0, typename R>
+__device__ __forceinline__ void dft12(cx<R> *a) {
+  const R sg = S < 0 ? R(-1) : R(1);
+  const R h = R(0.5), c = R(0.86602540378443864676372317075293618);
+  cx<R> b[3][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; n2++) {
+    if constexpr (MID0) {
+      const cx<R> x = a[n2], y = a[9 + n2];
+      b[n2][0] = add(x, y);
+      b[n2][2] = sub(x, y);
+      if constexpr (S < 0) {  // y1 = x + i y, y3 = x - i y
+        b[n2][1] = cx<R>{x.x - y.y, x.y + y.x};
+        b[n2][3] = cx<R>{x.x + y.y, x.y - y.x};
+      } else {  // y1 = x - i y, y3 = x + i y
+        b[n2][1] = cx<R>{x.x + y.y, x.y - y.x};
+        b[n2][3] = cx<R>{x.x - y.y, x.y + y.x};
+      }
+    } else {
+      cx<R> q[4] = {a[n2], a[3 + n2], a[6 + n2], a[9 + n2]};
+      dft<4, S>(q);
+#pragma unroll
+      for (int q1 = 0; q1 < 4; q1++) b[n2][q1] = q[q1];
+    }
+  }
+  // twiddles w12^(n2 q1): (n2,q1) = (1,1) (1,2) (1,3) (2,1) (2,2) (2,3)
+  b[1][1] = mul(b[1][1], cx<R>{c, sg * h});
+  b[1][2] = mul(b[1][2], cx<R>{h, sg * c});
+  b[1][3] = cx<R>{-sg * b[1][3].y, sg * b[1][3].x};  // * S i
+  b[2][1] = mul(b[2][1], cx<R>{h, sg * c});
+  b[2][2] = mul(b[2][2], cx<R>{-h, sg * c});
+  b[2][3] = cx<R>{-b[2][3].x, -b[2][3].y};  // * -1
+#pragma unroll
+  for (int q1 = 0; q1 < 4; q1++) {
+    cx<R> t[3] = {b[0][q1], b[1][q1], b[2][q1]};
+    dft<3, S>(t);
+#pragma unroll
+    for (int q2 = 0; q2 < 3; q2++) a[q1 + 4 * q2] = t[q2];
+  }
+}
+
 template <int RAD, int S, typename R>
 __device__ __forceinline__ void dft(cx<R> *a) {
   if constexpr (RAD == 2) {
@@ -165,32 +215,7 @@ __device__ __forceinline__ void dft(cx<R> *a) {
     a[1] = cx<R>{t2.x - s * d.y, t2.y + s * d.x};
     a[2] = cx<R>{t2.x + s * d.y, t2.y - s * d.x};
   } else if constexpr (RAD == 12) {
-    // 12 = 4 x 3: n = 3 n1 + n2, q = q1 + 4 q2;
-    // y_q = sum_n2 w3^(n2 q2) w12^(n2 q1) sum_n1 w4^(n1 q1) a_(3 n1 + n2)
-    const R sg = S < 0 ? R(-1) : R(1);
-    const R h = R(0.5), c = R(0.86602540378443864676372317075293618);
-    cx<R> b[3][4];
-#pragma unroll
-    for (int n2 = 0; n2 < 3; n2++) {
-      cx<R> q[4] = {a[n2], a[3 + n2], a[6 + n2], a[9 + n2]};
-      dft<4, S>(q);
-#pragma unroll
-      for (int q1 = 0; q1 < 4; q1++) b[n2][q1] = q[q1];
-    }
-    // twiddles w12^(n2 q1): (n2,q1) = (1,1) (1,2) (1,3) (2,1) (2,2) (2,3)
-    b[1][1] = mul(b[1][1], cx<R>{c, sg * h});
-    b[1][2] = mul(b[1][2], cx<R>{h, sg * c});
-    b[1][3] = cx<R>{-sg * b[1][3].y, sg * b[1][3].x};  // * S i
-    b[2][1] = mul(b[2][1], cx<R>{h, sg * c});
-    b[2][2] = mul(b[2][2], cx<R>{-h, sg * c});
-    b[2][3] = cx<R>{-b[2][3].x, -b[2][3].y};  // * -1
-#pragma unroll
-    for (int q1 = 0; q1 < 4; q1++) {
-      cx<R> t[3] = {b[0][q1], b[1][q1], b[2][q1]};
-      dft<3, S>(t);
-#pragma unroll
-      for (int q2 = 0; q2 < 3; q2++) a[q1 + 4 * q2] = t[q2];
-    }
+    dft12<S, false>(a);
   } else {
     static_assert(RAD == 16, "radix");
     // 16 = 4 x 4: n = 4 n1 + n2, q = q1 + 4 q2
@@ -464,11 +489,12 @@ __device__ __forceinline__ void fs_apply(cx<R> *v, const cx<R> *w) {
   for (int k = 1; k < n; k++) v[k] = S < 0 ? mul(v[k], w[k - 1]) : mulc(v[k], w[k - 1]);
 }
 
-template <typename R>
+// MID0: slots 3..8 hold zeros (see dft12)
+template <typename R, bool MID0 = false>
 __device__ __forceinline__ void fs768_bwd(cx<R> (&e)[12], cx<R> *lds, const FsTw<R> &w, int t) {
   const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
   cx<R> *row = lds + g * FS_RG;
-  dft<12, +1>(e);
+  dft12<+1, MID0>(e);
   fs_apply<+1, 12>(e, w.w768);
   fs_xpose(e);
 #pragma unroll

@@ -127,10 +127,9 @@ constexpr int xmodes() {
 // for t > 0 it is lane T-t, slot E-1-s; for t = 0 the thread itself, slot
 // (E-s) mod E.  Teams of one wavefront at most (the 128-lane teams of the
 // 1536-point rows combine their modes in fs1536_mode instead).
-template <int T, int E, typename R>
-__device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[xmodes<E>()], int t, int nmode) {
+template <int T, int E, int MO, typename R>
+__device__ __forceinline__ void mirror_pairs(const cx<R> (&e)[E], cx<R> (&m)[MO], int t, int nmode) {
   static_assert(T <= 64, "mirror_pairs works within one wavefront");
-  constexpr int MO = xmodes<E>();
   const int lane = (int)(threadIdx.x & 63);
   const int src = lane - t + ((T - t) % T);
 #pragma unroll
@@ -156,7 +155,11 @@ template <int M, typename R>
 constexpr int xteams() {
   return (M == 1536 || (M == 768 && sizeof(R) == 8)) ? 2 : ZB;
 }
-template <int M, typename R, bool PRODUCTS>
+// MO_: direct-mode slots per lane, xmodes<E>() or fewer when nx + 1 <= T MO_
+// (the launcher picks 3 for 768-point rows of at most 192 modes): the
+// compiler then knows the middle slots of every backward transform are zero
+// and drops the loads, selects and outputs that would only handle them
+template <int M, typename R, bool PRODUCTS, int MO_ = xmodes<fft::Plan<M>::E>()>
 __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
@@ -165,7 +168,8 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   using PL = fft::Plan<M>;
   constexpr int TB = xteams<M, R>();
   constexpr int T = PL::T, E = PL::E, NT = TB * T;
-  constexpr int MO = xmodes<E>();
+  constexpr int MO = MO_;
+  static_assert(MO <= xmodes<E>() && 2 * MO <= E, "direct and mirrored slots");
   // 768-point rows use the four-step transform (fs768_bwd/fwd), 1536-point
   // rows its two-wave form (fs1536_bwd/fwd); the samples between the two
   // directions sit in a permuted layout, which the pointwise CFL and products
@@ -207,7 +211,7 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   if constexpr (FS || FS2) fw = fft::fs_twiddles(twl, t & 63);
   auto bwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
-      fft::fs768_bwd(v, frc, fw, t);
+      fft::fs768_bwd<R, MO <= 3>(v, frc, fw, t);
     else if constexpr (FS2)
       fft::fs1536_bwd(v, frc, fw, twl, t);
     else
@@ -356,24 +360,26 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
       for (int s = 0; s < E; s++) {
         R a, b;
         if (pp == 0) {
-          a = u[s] * u[s] * factor;
-          b = u[s] * v[s] * factor;
+          a = u[s] * u[s];
+          b = u[s] * v[s];
         } else if (pp == 1) {
-          a = u[s] * w[s] * factor;
-          b = v[s] * v[s] * factor;
+          a = u[s] * w[s];
+          b = v[s] * v[s];
         } else {
-          a = v[s] * w[s] * factor;
-          b = w[s] * w[s] * factor;
+          a = v[s] * w[s];
+          b = w[s] * w[s];
         }
         e[s] = cx<R>{a, b};
       }
       fwd(e);
-      // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx
+      // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx,
+      // with the transform's normalisation factor applied here, once per mode
       cx<R> *da = B + ((size_t)(pl * 6 + qa) * NZB + zb) * Nx * ZB + zi;
       cx<R> *db = B + ((size_t)(pl * 6 + qb) * NZB + zb) * Nx * ZB + zi;
+      const R hf = R(0.5) * factor;
       auto put = [&](int k, cx<R> z, cx<R> q) {
-        da[(size_t)k * ZB] = cx<R>{(z.x + q.x) * R(0.5), (z.y - q.y) * R(0.5)};
-        db[(size_t)k * ZB] = cx<R>{(z.y + q.y) * R(0.5), (q.x - z.x) * R(0.5)};
+        da[(size_t)k * ZB] = cx<R>{(z.x + q.x) * hf, (z.y - q.y) * hf};
+        db[(size_t)k * ZB] = cx<R>{(z.y + q.y) * hf, (q.x - z.x) * hf};
       };
       if constexpr (FS2) {
         // wave h forms the modes 256 h .. 256 h + 255 (nx < 512 when M = 1536)
@@ -390,7 +396,7 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
         fft::team_sync<T>();  // the rows are the next transform's exchange area
       } else {
         cx<R> mz[MO];
-        mirror_pairs<T, E, R>(e, mz, t, nx + 1);
+        mirror_pairs<T, E, MO, R>(e, mz, t, nx + 1);
 #pragma unroll
         for (int s = 0; s < MO; s++) {
           const int k = t + T * s;
@@ -616,7 +622,10 @@ hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const c
   case N: {                                                                                                           \
     dim3 blk(xteams<N, R>() * fft::Plan<N>::T);                                                                       \
     dim3 grid(xteams<N, R>() < ZB ? 16 * ((ntile + 7) / 8) : ntile);                                                  \
-    if (products)                                                                                                     \
+    if (products && N == 768 && g.nx + 1 <= 192)                                                                      \
+      hipLaunchKernelGGL((k_xpass<N, R, true, (N == 768 ? 3 : xmodes<fft::Plan<N>::E>())>), grid, blk, 0, st, A, B, tw, \
+                         g, plane0, cfl_on, dy, dx, dz, factor, cfl_bits, nplanes);                                   \
+    else if (products)                                                                                                \
       hipLaunchKernelGGL((k_xpass<N, R, true>), grid, blk, 0, st, A, B, tw, g, plane0, cfl_on, dy, dx, dz, factor,    \
                          cfl_bits, nplanes);                                                                          \
     else                                                                                                              \
