@@ -567,6 +567,129 @@ __device__ __forceinline__ void fs768_fwd(cx<R> (&e)[12], cx<R> *lds, const FsTw
   dft<12, -1>(e);
 }
 
+// Two independent transforms of one wave in an interleaved order: each LDS
+// exchange of one transform is issued, then the other transform's next
+// register stage runs while the exchange's reads are in flight.  One exchange
+// area serves both, as the wave's LDS operations execute in order (a read of
+// one exchange is done before the next exchange's writes land).  The same
+// arithmetic as fs768_bwd / fs768_fwd on each array, so the results are
+// bitwise those of two calls.
+template <typename R, bool MID0 = false>
+__device__ __forceinline__ void fs768_bwd2(cx<R> (&e)[12], cx<R> (&f)[12], cx<R> *lds, const FsTw<R> &w, int t) {
+  const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
+  cx<R> *row = lds + g * FS_RG;
+  auto stage_a = [&](cx<R>(&x)[12]) {
+    dft12<+1, MID0>(x);
+    fs_apply<+1, 12>(x, w.w768);
+    fs_xpose(x);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      dft<4, +1>(&x[4 * q]);
+      fs_apply<+1, 4>(&x[4 * q], w.w64);
+    }
+  };
+  auto xch1 = [&](const cx<R>(&x)[12], cx<R>(&v)[12]) {
+#pragma unroll
+    for (int xi = 0; xi < 12; xi++) row[c * 52 + xi * 4 + d] = x[xi];
+    team_sync<64>();
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int cp = 0; cp < 4; cp++) v[4 * i + cp] = row[cp * 52 + (3 * c + i) * 4 + d];
+    team_sync<64>();
+  };
+  auto stage_b = [&](cx<R>(&v)[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      dft<4, +1>(&v[4 * i]);
+      fs_apply<+1, 4>(&v[4 * i], w.w16);
+    }
+  };
+  auto xch2 = [&](const cx<R>(&v)[12], cx<R>(&x)[12]) {
+#pragma unroll
+    for (int eta = 0; eta < 12; eta++) row[c * 12 + d * 49 + eta] = v[eta];
+    team_sync<64>();
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int dp = 0; dp < 4; dp++) x[4 * j + dp] = row[c * 12 + dp * 49 + 3 * d + j];
+    team_sync<64>();
+  };
+  auto stage_c = [&](cx<R>(&x)[12]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) dft<4, +1>(&x[4 * j]);
+  };
+  cx<R> ve[12], vf[12];
+  stage_a(e);
+  xch1(e, ve);
+  stage_a(f);  // while e's reads are in flight
+  xch1(f, vf);
+  stage_b(ve);
+  xch2(ve, e);
+  stage_b(vf);
+  xch2(vf, f);
+  stage_c(e);
+  stage_c(f);
+}
+
+template <typename R>
+__device__ __forceinline__ void fs768_fwd2(cx<R> (&e)[12], cx<R> (&f)[12], cx<R> *lds, const FsTw<R> &w, int t) {
+  const int p = t & 15, g = t >> 4, d = p & 3, c = p >> 2;
+  cx<R> *row = lds + g * FS_RG;
+  auto stage_c = [&](cx<R>(&x)[12]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) dft<4, -1>(&x[4 * j]);
+  };
+  auto xch2 = [&](const cx<R>(&x)[12], cx<R>(&v)[12]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int dp = 0; dp < 4; dp++) row[c * 12 + dp * 49 + 3 * d + j] = x[4 * j + dp];
+    team_sync<64>();
+#pragma unroll
+    for (int eta = 0; eta < 12; eta++) v[eta] = row[c * 12 + d * 49 + eta];
+    team_sync<64>();
+  };
+  auto stage_b = [&](cx<R>(&v)[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      fs_apply<-1, 4>(&v[4 * i], w.w16);
+      dft<4, -1>(&v[4 * i]);
+    }
+  };
+  auto xch1 = [&](const cx<R>(&v)[12], cx<R>(&x)[12]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int cp = 0; cp < 4; cp++) row[cp * 52 + (3 * c + i) * 4 + d] = v[4 * i + cp];
+    team_sync<64>();
+#pragma unroll
+    for (int xi = 0; xi < 12; xi++) x[xi] = row[c * 52 + xi * 4 + d];
+    team_sync<64>();
+  };
+  auto stage_a = [&](cx<R>(&x)[12]) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      fs_apply<-1, 4>(&x[4 * q], w.w64);
+      dft<4, -1>(&x[4 * q]);
+    }
+    fs_xpose(x);
+    fs_apply<-1, 12>(x, w.w768);
+    dft<12, -1>(x);
+  };
+  cx<R> ve[12], vf[12];
+  stage_c(e);
+  xch2(e, ve);
+  stage_c(f);  // while e's reads are in flight
+  xch2(f, vf);
+  stage_b(ve);
+  xch1(ve, e);
+  stage_b(vf);
+  xch1(vf, f);
+  stage_a(e);
+  stage_a(f);
+}
+
 // ---------------------------------------------------- two-wave 1536 --
 // A 1536-point transform held by a 128-lane team of two wavefronts, 12 values
 // per lane: one radix-2 step between the waves, then the four-step 768-point

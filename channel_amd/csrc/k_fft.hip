@@ -169,6 +169,10 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   constexpr int TB = xteams<M, R>();
   constexpr int T = PL::T, E = PL::E, NT = TB * T;
   constexpr int MO = MO_;
+  // FP64 768-point rows run their transforms in interleaved pairs
+  // (fs768_bwd2 / fs768_fwd2): at two waves per SIMD the single transforms
+  // leave the VALU waiting on their LDS exchanges
+  constexpr bool ILV = M == 768 && sizeof(R) == 8;
   static_assert(MO <= xmodes<E>() && 2 * MO <= E, "direct and mirrored slots");
   // 768-point rows use the four-step transform (fs768_bwd/fwd), 1536-point
   // rows its two-wave form (fs1536_bwd/fwd); the samples between the two
@@ -309,20 +313,24 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
     }
 #pragma unroll
     for (int s = 0; s < E; s++) e[s] = cx<R>{e[s].x - f[s].y, e[s].y + f[s].x};  // U + iV
-    if constexpr (!SHFL) load_raw(2, f);  // in flight during the first transform
-    bwd(e);
+    if constexpr (ILV) {
+      extend(rw, f);
+      fft::fs768_bwd2<R, MO <= 3>(e, f, frc, fw, t);
+    } else {
+      if constexpr (!SHFL) load_raw(2, f);  // in flight during the first transform
+      bwd(e);
+      if constexpr (SHFL)
+        extend(rw, f);
+      else
+        fix_ext(f);
+      bwd(f);
+    }
 #pragma unroll
     for (int s = 0; s < E; s++) {
       u[s] = e[s].x;
       v[s] = e[s].y;
+      w[s] = f[s].x;
     }
-    if constexpr (SHFL)
-      extend(rw, f);
-    else
-      fix_ext(f);
-    bwd(f);
-#pragma unroll
-    for (int s = 0; s < E; s++) w[s] = f[s].x;
   }
 
   // CFL maximum over the interior planes (dnsdata.f90:552-556).  FP64 keeps
@@ -350,12 +358,9 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
 
   if constexpr (PRODUCTS) {
     // pairs (uu,uv), (uw,vv), (vw,ww) share one forward transform each; in
-    // this order u is dead once the second pair is formed.  Product indices:
-    // 0 uu, 1 vv, 2 ww, 3 uv, 4 vw, 5 uw.
-#pragma unroll
-    for (int pp = 0; pp < 3; pp++) {
-      const int qa = pp == 0 ? 0 : (pp == 1 ? 5 : 4), qb = pp == 0 ? 3 : (pp == 1 ? 1 : 2);
-      cx<R> e[E];
+    // this order u is dead once the second pair is formed, and v and w once
+    // the third is.  Product indices: 0 uu, 1 vv, 2 ww, 3 uv, 4 vw, 5 uw.
+    auto form = [&](const int pp, cx<R>(&e)[E]) {
 #pragma unroll
       for (int s = 0; s < E; s++) {
         R a, b;
@@ -371,9 +376,11 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
         }
         e[s] = cx<R>{a, b};
       }
-      fwd(e);
-      // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx,
-      // with the transform's normalisation factor applied here, once per mode
+    };
+    // A_k = (Z_k + conj Z_{M-k})/2, B_k = (Z_k - conj Z_{M-k})/(2i), k=0..nx,
+    // with the transform's normalisation factor applied here, once per mode
+    auto emit = [&](const int pp, const cx<R>(&e)[E]) {
+      const int qa = pp == 0 ? 0 : (pp == 1 ? 5 : 4), qb = pp == 0 ? 3 : (pp == 1 ? 1 : 2);
       cx<R> *da = B + ((size_t)(pl * 6 + qa) * NZB + zb) * Nx * ZB + zi;
       cx<R> *db = B + ((size_t)(pl * 6 + qb) * NZB + zb) * Nx * ZB + zi;
       const R hf = R(0.5) * factor;
@@ -402,6 +409,30 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
           const int k = t + T * s;
           if (k <= nx) put(k, e[s], mz[s]);
         }
+      }
+    };
+    if constexpr (ILV) {
+      // the first pair alone while u, v, w are live, then the other two
+      // together
+      {
+        cx<R> e[E];
+        form(0, e);
+        fwd(e);
+        emit(0, e);
+      }
+      cx<R> e[E], f[E];
+      form(1, e);
+      form(2, f);
+      fft::fs768_fwd2(e, f, frc, fw, t);
+      emit(1, e);
+      emit(2, f);
+    } else {
+#pragma unroll
+      for (int pp = 0; pp < 3; pp++) {
+        cx<R> e[E];
+        form(pp, e);
+        fwd(e);
+        emit(pp, e);
       }
     }
   }
