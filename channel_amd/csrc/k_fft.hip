@@ -205,14 +205,22 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   cx<R> *frc = reinterpret_cast<cx<R> *>(fr);
 
   // twiddles in LDS: their reads then wait on the LDS counter, not behind
-  // the row loads still in flight on the vector-memory counter
-  constexpr int NTW = FS ? fft::FS_TW : (FS2 ? fft::FS2_TW : fft::plan_tw_size(M, XBIG));
+  // the row loads still in flight on the vector-memory counter.  The
+  // 768-point rows need only three table entries per lane, which they read
+  // from global memory before their row loads: no staging and no block
+  // barrier before the loads are issued.
+  constexpr int NTW = FS ? 0 : (FS2 ? fft::FS2_TW : fft::plan_tw_size(M, XBIG));
   __shared__ __align__(16) cx<R> twl[NTW > 0 ? NTW : 1];
-  for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
-  __syncthreads();
+  if constexpr (NTW > 0) {
+    for (int i = tid; i < NTW; i += NT) twl[i] = tw[i];
+    __syncthreads();
+  }
   // the four-step transform's twiddle powers, shared by the row's five transforms
   fft::FsTw<R> fw;
-  if constexpr (FS || FS2) fw = fft::fs_twiddles(twl, t & 63);
+  if constexpr (FS)
+    fw = fft::fs_twiddles(tw, t & 63);
+  else if constexpr (FS2)
+    fw = fft::fs_twiddles(twl, t & 63);
   auto bwd = [&](cx<R>(&v)[E]) {
     if constexpr (FS)
       fft::fs768_bwd<R, MO <= 3>(v, frc, fw, t);
