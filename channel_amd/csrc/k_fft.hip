@@ -101,20 +101,41 @@ constexpr int XXCH = fft::XCH_CPLX | fft::XCH_PAD;
 
 template <int NT>
 __device__ __forceinline__ void block_max_atomic(double m, double *red, unsigned long long *out) {
-  red[threadIdx.x] = m;
-  __syncthreads();
+  // a maximum that keeps a NaN
+  auto mx = [](double c, double o) { return (o > c || o != o) ? o : c; };
+  double r;
+  if constexpr (NT % 64 == 0) {
+    // within each wave by lane exchanges, then across the waves in LDS
 #pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double o = red[threadIdx.x + s];
-      const double c = red[threadIdx.x];
-      red[threadIdx.x] = (o > c || o != o) ? o : c;
-    }
+    for (int s = 32; s > 0; s >>= 1) m = mx(m, __shfl_xor(m, s, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
+    r = red[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) r = mx(r, red[w]);
+  } else {
+    red[threadIdx.x] = m;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = mx(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    r = red[0];
   }
   // values are >= 0 or NaN, so their bit patterns order as unsigned
   // integers (NaN above +Inf above every finite value)
-  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+  if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(r));
+}
+
+// x / d as the division rounds it, from r = 1/d rounded to nearest: q = x r
+// and one correction by the exact remainder x - q d.  For finite x and
+// normal d the corrected quotient is the correctly rounded one (Markstein's
+// theorem: r within half an ulp of 1/d, q within an ulp of x/d); an
+// infinite x gives NaN, which the CFL check reports as it reports Inf.
+__device__ __forceinline__ double div_by(double x, double d, double r) {
+  const double q = x * r;
+  return fma(fma(-q, d, x), r, q);
 }
 
 // x modes per thread of the x pass: nxd >= 3(nx+1)/2 bounds Nx by M/3
@@ -182,7 +203,7 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   constexpr int TEAM_R = FS ? 2 * fft::FS_LDS : (FS2 ? 4 * fft::FS_LDS : fft::lds_reals<M, XXCH>());
   static_assert(!FS2 || (T == 128 && E == 12 && MO == 4), "the two-wave 1536 transform needs 128-lane teams");
   __shared__ __align__(16) R fl[TB * TEAM_R];
-  __shared__ double red[NT];
+  __shared__ double red[NT % 64 == 0 ? NT / 64 : NT];
   const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
   const int tid = threadIdx.x, t = tid % T;
   int zb, pl, zi;
@@ -341,14 +362,15 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
     }
   }
 
-  // CFL maximum over the interior planes (dnsdata.f90:552-556).  FP64 keeps
-  // the reference's divisions; the split mode, whose samples are FP32, takes
-  // the three reciprocals once per row (within an FP64 rounding of them).
+  // CFL maximum over the interior planes (dnsdata.f90:552-556).  FP64 gives
+  // the reference's quotients (div_by: the correctly rounded divisions from
+  // three reciprocals per row); the split mode, whose samples are FP32,
+  // multiplies by the reciprocals (within an FP64 rounding of the quotients).
   if (cfl_on && iy >= 1 && iy <= g.ny - 1) {
     const double dyv = dy[iy];
+    const double rx = 1.0 / dx, ry = 1.0 / dyv, rz = 1.0 / dz;
     double m = 0.0;
     if constexpr (sizeof(R) == 4) {
-      const double rx = 1.0 / dx, ry = 1.0 / dyv, rz = 1.0 / dz;
 #pragma unroll
       for (int s = 0; s < E; s++) {
         const double c = fabs((double)u[s]) * rx + fabs((double)v[s]) * ry + fabs((double)w[s]) * rz;
@@ -357,7 +379,8 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
     } else {
 #pragma unroll
       for (int s = 0; s < E; s++) {
-        const double c = fabs((double)u[s]) / dx + fabs((double)v[s]) / dyv + fabs((double)w[s]) / dz;
+        const double c = div_by(fabs((double)u[s]), dx, rx) + div_by(fabs((double)v[s]), dyv, ry) +
+                         div_by(fabs((double)w[s]), dz, rz);
         m = (c > m || c != c) ? c : m;
       }
     }
