@@ -180,7 +180,9 @@ constexpr int xteams() {
 // (the launcher picks 3 for 768-point rows of at most 192 modes): the
 // compiler then knows the middle slots of every backward transform are zero
 // and drops the loads, selects and outputs that would only handle them
-template <int M, typename R, bool PRODUCTS, int MO_ = xmodes<fft::Plan<M>::E>()>
+// FULL: nx + 1 == T MO_ exactly (nx = 191 for 768-point rows), so the mode
+// bounds are compile-time constants and their selects and branches fold away
+template <int M, typename R, bool PRODUCTS, int MO_ = xmodes<fft::Plan<M>::E>(), bool FULL = false>
 __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass(const cx<R> *__restrict__ A, cx<R> *__restrict__ B,
                                                                 const cx<R> *__restrict__ tw, Geom g, int plane0,
                                                                 int cfl_on, const double *__restrict__ dy, double dx,
@@ -204,7 +206,7 @@ __global__ __launch_bounds__((xteams<M, R>() * fft::Plan<M>::T), 2) void k_xpass
   static_assert(!FS2 || (T == 128 && E == 12 && MO == 4), "the two-wave 1536 transform needs 128-lane teams");
   __shared__ __align__(16) R fl[TB * TEAM_R];
   __shared__ double red[NT % 64 == 0 ? NT / 64 : NT];
-  const int Nx = g.Nx, nx = g.nx, NZB = g.nzd / ZB;
+  const int Nx = g.Nx, nx = FULL ? fft::Plan<M>::T * MO_ - 1 : g.nx, NZB = g.nzd / ZB;
   const int tid = threadIdx.x, t = tid % T;
   int zb, pl, zi;
   if constexpr (TB < ZB) {
@@ -684,7 +686,10 @@ hipError_t launch_xpass(int M, hipStream_t st, const cx<R> *A, cx<R> *B, const c
   case N: {                                                                                                           \
     dim3 blk(xteams<N, R>() * fft::Plan<N>::T);                                                                       \
     dim3 grid(xteams<N, R>() < ZB ? 16 * ((ntile + 7) / 8) : ntile);                                                  \
-    if (products && N == 768 && g.nx + 1 <= 192)                                                                      \
+    if (products && N == 768 && g.nx + 1 == 192)                                                                      \
+      hipLaunchKernelGGL((k_xpass<N, R, true, (N == 768 ? 3 : xmodes<fft::Plan<N>::E>()), true>), grid, blk, 0, st, A, \
+                         B, tw, g, plane0, cfl_on, dy, dx, dz, factor, cfl_bits, nplanes);                            \
+    else if (products && N == 768 && g.nx + 1 <= 192)                                                                 \
       hipLaunchKernelGGL((k_xpass<N, R, true, (N == 768 ? 3 : xmodes<fft::Plan<N>::E>())>), grid, blk, 0, st, A, B, tw, \
                          g, plane0, cfl_on, dy, dx, dz, factor, cfl_bits, nplanes);                                   \
     else if (products)                                                                                                \

@@ -8,8 +8,13 @@ the field in the reference layout: every row of the modes |kx|, |kz| <= 2
 (index ix <= 2, |iz| <= 2) and 4096 entries drawn uniformly (numpy seed 5);
 'v1', 'v10' -- those entries after steps 1 and 10 (complex128).
 
+With a step count above 10 (c3_record_100.npz) the run goes on to that
+step, 'rows' holds every row reached, 'v20', 'v50', 'v100' the entries after
+those steps, and the file is rewritten at each of them with 'last_step', so
+that a run cut short keeps what it reached.
+
 The oracle needs about 27 s per step on 16 cores; run from the repository
-root: python tests/golden/make_c3_record.py [threads] [out.npz]
+root: python tests/golden/make_c3_record.py [threads] [out.npz] [steps]
 """
 import os
 import sys
@@ -43,6 +48,7 @@ def main():
     from oracle import oracle as orc
     threads = int(sys.argv[1]) if len(sys.argv) > 1 else (os.cpu_count() or 1)
     out = sys.argv[2] if len(sys.argv) > 2 else os.path.join(os.path.dirname(os.path.abspath(__file__)), "c3_record.npz")
+    nsteps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
     orc.set_threads(threads)
     cfg = config3()
     idx = sample_index(cfg)
@@ -52,12 +58,16 @@ def main():
     rows = [o.outstats()]
     res = {}
     t0 = time.time()
-    for s in range(1, 11):
+    marks = (1, 10) if nsteps <= 10 else (1, 10, 20, 50, 100, nsteps)
+    for s in range(1, nsteps + 1):
         rows.append(o.step())
-        if s in (1, 10):
+        if s in marks:
             res["v%d" % s] = take(o.get_field(), idx)
+            if nsteps > 10:
+                np.savez_compressed(out, rows=np.array(rows), idx=idx.astype(np.int16), last_step=np.array(s), **res)
         print("step", s, "%.0f s" % (time.time() - t0), flush=True)
-    np.savez_compressed(out, rows=np.array(rows), idx=idx.astype(np.int16), **res)
+    if nsteps <= 10:
+        np.savez_compressed(out, rows=np.array(rows), idx=idx.astype(np.int16), **res)
 
 
 if __name__ == "__main__":
