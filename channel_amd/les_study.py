@@ -1,8 +1,9 @@
-"""The learned subgrid-stress closure on a coarse channel grid, judged by
-the filtered fine DNS (the closure has no counterpart in the reference).
+"""The learned subgrid-stress closure on a coarse channel grid, measured
+against the filtered fine DNS (the closure has no counterpart in the
+reference).
 
 Fine DNS: the production grid 191x384x189 at Re 12431, alfa0 0.5, beta0 1,
-CPI type 1 (gamma 0.161436), cflmax 1 -- this build's own GPU step, from a
+CPI type 1 (gamma 0.161436), cflmax 1 -- the library's own GPU step, from a
 strongly perturbed laminar start.  Coarse (LES) grid: nx = 31, ny = 64,
 nz = 31 with the same physics: its dealiased physical grid 96 x 96 is the
 fine 768 x 768 divided by 8, and its stretched rows are every 6th fine row
