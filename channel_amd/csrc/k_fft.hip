@@ -4,8 +4,8 @@
 //   K1 k_zinv : V(iy,iz,ix,c) -> zero-pad z to nzd, backward c2c FFT in z
 //               (IFT, ffts.f90:93-96)  -> A[plane][c][zb][ix][ZB]
 //   K2 k_xpass: per physical-z row: Hermitian c2r in x (RFT, zero modes
-//               nx+1..nxd), CFL maximum, the six products times `factor`,
-//               r2c in x keeping modes 0..nx (HFT)      -> B[plane][q][zb][ix][ZB]
+//               nx+1..nxd), CFL maximum, the six products, r2c in x keeping
+//               modes 0..nx (HFT), times `factor`       -> B[plane][q][zb][ix][ZB]
 //   K3 k_zfwd : forward c2c FFT in z, keep modes -nz..nz (FFT + izd)
 //                                                       -> P[q][plane][pair]
 //
