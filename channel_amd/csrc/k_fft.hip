@@ -36,11 +36,18 @@ template <int NZ>
 constexpr int stage_stride() {
   return NZ + 4;
 }
+// FP64 blocks stage the real and the imaginary parts in two rounds through
+// one real-valued area: half the LDS of whole complex values, which would
+// hold an FP64 block to three waves per SIMD
+template <typename R>
+constexpr bool z_split_stage() {
+  return sizeof(R) == 8;
+}
 template <int NZ, typename R>
 constexpr int zbuf_reals() {
   constexpr int G = 256 / fft::Plan<NZ>::T;
   constexpr int fftr = G * fft::lds_reals<NZ, ZXCH>();
-  constexpr int stg = z_staged<NZ, R>() ? 2 * G * stage_stride<NZ>() : 0;
+  constexpr int stg = z_staged<NZ, R>() ? (z_split_stage<R>() ? 1 : 2) * G * stage_stride<NZ>() : 0;
   return fftr > stg ? fftr : stg;
 }
 
@@ -70,7 +77,29 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
   }
   fft::fft_slots<NZ, +1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
   cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
-  if constexpr (z_staged<NZ, R>()) {
+  if constexpr (z_staged<NZ, R>() && z_split_stage<R>()) {
+    const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
+    R out[2][E];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      __syncthreads();  // the transforms' exchanges, or the previous round's reads, are done
+#pragma unroll
+      for (int s = 0; s < E; s++) lds[team * SP + t + T * s] = h ? e[s].y : e[s].x;
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const int el = (int)threadIdx.x + 256 * i;  // G*NZ = 256*E elements per block
+        const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
+        out[h][i] = lds[min(tm, nact - 1) * SP + zb * ZB + r];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int el = (int)threadIdx.x + 256 * i;
+      const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB;
+      if (tm < nact) dst[((size_t)zb * g.Nx + ix0) * ZB + w] = cx<R>{out[0][i], out[1][i]};
+    }
+  } else if constexpr (z_staged<NZ, R>()) {
     cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
     __syncthreads();  // every team is done with its transform exchanges
 #pragma unroll
@@ -489,7 +518,6 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
   const cx<R> *src = B + (size_t)(pl * 6 + q) * NZ * g.Nx;
   cx<R> e[E];
   if constexpr (z_staged<NZ, R>()) {
-    cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
     const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
     cx<R> v[E];
 #pragma unroll
@@ -498,16 +526,33 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
       const int zb = el / (G * ZB), w = el % (G * ZB);
       v[i] = src[((size_t)zb * g.Nx + ix0) * ZB + min(w, nact * ZB - 1)];
     }
+    if constexpr (z_split_stage<R>()) {
 #pragma unroll
-    for (int i = 0; i < E; i++) {
-      const int el = (int)threadIdx.x + 256 * i;
-      const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
-      stg[tm * SP + zb * ZB + r] = tm < nact ? v[i] : cx<R>{0, 0};
+      for (int h = 0; h < 2; h++) {
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+          const int el = (int)threadIdx.x + 256 * i;
+          const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
+          lds[tm * SP + zb * ZB + r] = tm < nact ? (h ? v[i].y : v[i].x) : R(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < E; s++) (h ? e[s].y : e[s].x) = lds[team * SP + t + T * s];
+        __syncthreads();  // the next round and the transforms reuse the staging area
+      }
+    } else {
+      cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
+#pragma unroll
+      for (int i = 0; i < E; i++) {
+        const int el = (int)threadIdx.x + 256 * i;
+        const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
+        stg[tm * SP + zb * ZB + r] = tm < nact ? v[i] : cx<R>{0, 0};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < E; s++) e[s] = stg[team * SP + t + T * s];
+      __syncthreads();  // the transforms reuse the staging area
     }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < E; s++) e[s] = stg[team * SP + t + T * s];
-    __syncthreads();  // the transforms reuse the staging area
   } else {
 #pragma unroll
     for (int s = 0; s < E; s++) {
