@@ -22,10 +22,10 @@ namespace chan {
 // their LDS footprint small
 constexpr int ZXCH = fft::XCH_PAD;
 
-// FP32 z blocks are 32 bytes per x mode, too short for efficient memory
-// transactions: in FP32 the z kernels stage through LDS, where the G x
-// modes of a block form one contiguous run of G*ZB values per z block, and
-// lanes move whole runs (a wave touches 4 runs of 128 bytes).
+// A z block is 32 (FP32) or 64 (FP64) bytes per x mode, short for memory
+// transactions: the z kernels stage through LDS, where the G x modes of a
+// block form one contiguous run of G*ZB values per z block, and lanes move
+// whole runs (a wave touches 4 runs of 128 or 256 bytes).
 template <int NZ, typename R>
 constexpr bool z_staged() {
   return true;
@@ -36,18 +36,16 @@ template <int NZ>
 constexpr int stage_stride() {
   return NZ + 4;
 }
-// FP64 blocks stage the real and the imaginary parts in two rounds through
+// The blocks stage the real and the imaginary parts in two rounds through
 // one real-valued area: half the LDS of whole complex values, which would
-// hold an FP64 block to three waves per SIMD
-template <typename R>
-constexpr bool z_split_stage() {
-  return sizeof(R) == 8;
-}
+// hold an FP64 block to three waves per SIMD and an FP32 block to six (four
+// and eight with the two rounds: the FP64 step 1.5% and the FP32 z kernels
+// 4-9% faster)
 template <int NZ, typename R>
 constexpr int zbuf_reals() {
   constexpr int G = 256 / fft::Plan<NZ>::T;
   constexpr int fftr = G * fft::lds_reals<NZ, ZXCH>();
-  constexpr int stg = z_staged<NZ, R>() ? (z_split_stage<R>() ? 1 : 2) * G * stage_stride<NZ>() : 0;
+  constexpr int stg = z_staged<NZ, R>() ? G * stage_stride<NZ>() : 0;
   return fftr > stg ? fftr : stg;
 }
 
@@ -77,7 +75,7 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
   }
   fft::fft_slots<NZ, +1, true, ZXCH>(e, lds + team * fft::lds_reals<NZ, ZXCH>(), tw, t);
   cx<R> *dst = A + (size_t)(pl * 3 + c) * NZ * g.Nx;
-  if constexpr (z_staged<NZ, R>() && z_split_stage<R>()) {
+  if constexpr (z_staged<NZ, R>()) {
     const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
     R out[2][E];
 #pragma unroll
@@ -98,19 +96,6 @@ __global__ __launch_bounds__(256) void k_zinv(const cx<double> *__restrict__ V, 
       const int el = (int)threadIdx.x + 256 * i;
       const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB;
       if (tm < nact) dst[((size_t)zb * g.Nx + ix0) * ZB + w] = cx<R>{out[0][i], out[1][i]};
-    }
-  } else if constexpr (z_staged<NZ, R>()) {
-    cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
-    __syncthreads();  // every team is done with its transform exchanges
-#pragma unroll
-    for (int s = 0; s < E; s++) stg[team * SP + t + T * s] = e[s];
-    __syncthreads();
-    const int ix0 = xg * G, nact = min(G, g.Nx - ix0);
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-      const int el = (int)threadIdx.x + 256 * i;  // G*NZ = 256*E elements per block
-      const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
-      if (tm < nact) dst[((size_t)zb * g.Nx + ix0) * ZB + w] = stg[tm * SP + zb * ZB + r];
     }
   } else if (act) {
 #pragma unroll
@@ -526,32 +511,18 @@ __global__ __launch_bounds__(256) void k_zfwd(const cx<R> *__restrict__ B, cx<R>
       const int zb = el / (G * ZB), w = el % (G * ZB);
       v[i] = src[((size_t)zb * g.Nx + ix0) * ZB + min(w, nact * ZB - 1)];
     }
-    if constexpr (z_split_stage<R>()) {
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-#pragma unroll
-        for (int i = 0; i < E; i++) {
-          const int el = (int)threadIdx.x + 256 * i;
-          const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
-          lds[tm * SP + zb * ZB + r] = tm < nact ? (h ? v[i].y : v[i].x) : R(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < E; s++) (h ? e[s].y : e[s].x) = lds[team * SP + t + T * s];
-        __syncthreads();  // the next round and the transforms reuse the staging area
-      }
-    } else {
-      cx<R> *stg = reinterpret_cast<cx<R> *>(lds);
+    for (int h = 0; h < 2; h++) {
 #pragma unroll
       for (int i = 0; i < E; i++) {
         const int el = (int)threadIdx.x + 256 * i;
         const int zb = el / (G * ZB), w = el % (G * ZB), tm = w / ZB, r = w % ZB;
-        stg[tm * SP + zb * ZB + r] = tm < nact ? v[i] : cx<R>{0, 0};
+        lds[tm * SP + zb * ZB + r] = tm < nact ? (h ? v[i].y : v[i].x) : R(0);
       }
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < E; s++) e[s] = stg[team * SP + t + T * s];
-      __syncthreads();  // the transforms reuse the staging area
+      for (int s = 0; s < E; s++) (h ? e[s].y : e[s].x) = lds[team * SP + t + T * s];
+      __syncthreads();  // the next round and the transforms reuse the staging area
     }
   } else {
 #pragma unroll
